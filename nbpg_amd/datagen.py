@@ -1,0 +1,71 @@
+"""Seeded synthetic workloads for the NBPG hot path.
+
+The reference's benchmarks use SIFT1M downloaded over ftp; offline, every
+workload here is a seeded Gaussian-cluster mixture of the same shape.  All
+draws come from numpy's PCG64 with fixed seeds, so the vectors are identical
+on every host.
+"""
+import numpy as np
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def cluster_centers_u8(n_clusters, dims, seed, lo=20.0, hi=200.0):
+    return _rng(seed).uniform(lo, hi, size=(n_clusters, dims)).astype(np.float32)
+
+
+def mixture_u8(n, dims, n_clusters, sigma, seed, centers_seed=None, scale=None,
+               chunk=1 << 18):
+    """u8 rows: round(center + sigma * N(0,1) * scale) clipped to 0..255.
+
+    `scale` is an optional per-dimension multiplier (heavy-tailed workloads);
+    centres come from `centers_seed` so base and queries share them."""
+    centers = cluster_centers_u8(n_clusters, dims, centers_seed if centers_seed is not None else 7)
+    g = _rng(seed)
+    out = np.empty((n, dims), np.uint8)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        lab = g.integers(0, n_clusters, size=e - s)
+        noise = g.standard_normal(size=(e - s, dims), dtype=np.float32) * np.float32(sigma)
+        x = centers[lab] + noise
+        if scale is not None:
+            x = (x - 110.0) * scale + 110.0
+        np.clip(np.rint(x), 0, 255, out=x)
+        out[s:e] = x.astype(np.uint8)
+    return out
+
+
+def heavy_tail_scale(dims, seed=11, sigma=0.5):
+    """Per-dimension lognormal scale used by the 1M x 128 workload."""
+    return _rng(seed).lognormal(0.0, sigma, size=dims).astype(np.float32) * np.float32(1.6)
+
+
+def mixture_i8(n, dims, n_clusters, sigma, seed, centers_seed=5, chunk=1 << 18):
+    """i8 rows in -127..127 from a cluster mixture."""
+    centers = _rng(centers_seed).uniform(-90.0, 90.0, size=(n_clusters, dims)).astype(np.float32)
+    g = _rng(seed)
+    out = np.empty((n, dims), np.int8)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        lab = g.integers(0, n_clusters, size=e - s)
+        x = centers[lab] + g.standard_normal(size=(e - s, dims), dtype=np.float32) * np.float32(sigma)
+        np.clip(np.rint(x), -127, 127, out=x)
+        out[s:e] = x.astype(np.int8)
+    return out
+
+
+def workload_small(n=100_000, nq=1_000, dims=128):
+    """100k x 128 u8, 100 isotropic clusters (centres U[20,200], sigma 25)."""
+    base = mixture_u8(n, dims, 100, 25.0, seed=1, centers_seed=7)
+    queries = mixture_u8(nq, dims, 100, 25.0, seed=2, centers_seed=7)
+    return base, queries
+
+
+def workload_1m(n=1_000_000, nq=10_000, dims=128):
+    """1M x 128 u8, 1000 clusters, heavy-tailed per-dimension scale."""
+    sc = heavy_tail_scale(dims)
+    base = mixture_u8(n, dims, 1000, 25.0, seed=3, centers_seed=9, scale=sc)
+    queries = mixture_u8(nq, dims, 1000, 25.0, seed=4, centers_seed=9, scale=sc)
+    return base, queries
