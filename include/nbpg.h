@@ -1,0 +1,159 @@
+/*
+ * nbpg.h -- C-ABI of the MI355X (gfx950) NBPG hot path.
+ *
+ * One shared library (nbpg_amd/libnbpg_amd.so), HIP only.  Every compute
+ * entry point takes DEVICE pointers for its inputs and outputs and an
+ * optional hipStream_t (passed as void*, NULL = the legacy default stream),
+ * and returns an int status (NBPG_OK or a negative code; never aborts).
+ * nbpg_last_error() gives the message of the calling thread's last failure.
+ *
+ * Citations are file:line in the reference tree (godpgf/NBPG), naming the
+ * interface each entry point stands in for.
+ */
+#ifndef NBPG_H
+#define NBPG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference aborts instead: beamSearch.h:268-273,
+ *      graph.h:381-387, query_params.h:296-300) ---- */
+#define NBPG_OK 0
+#define NBPG_EINVAL -1   /* bad argument: k > L, id out of range, ... */
+#define NBPG_ENOMEM -2   /* device allocation failed */
+#define NBPG_EHIP -3     /* a HIP runtime call failed */
+#define NBPG_EUNSUP -4   /* shape the kernels do not support */
+
+/* ---- element types; the reference picks them by file extension
+ *      (Vamana.cpp:292-311, CompGT.cpp:152-163).  dims counts elements,
+ *      i.e. BYTES for the packed 4-bit types (point.h:167-169). ---- */
+#define NBPG_U8 0
+#define NBPG_I8 1
+#define NBPG_U4X2 2 /* .u4bin: two 4-bit codes per byte */
+#define NBPG_I4X2 3 /* .i4bin: distance identical to U4X2 (point.h:107-120) */
+
+/* ---- search modes ---- */
+#define NBPG_SEARCH_REF 0  /* one query per workgroup, reference-exact IDs */
+#define NBPG_SEARCH_FAST 1 /* persistent, one wavefront per query, LDS filter + beam */
+
+/* ---- CompGT modes ---- */
+#define NBPG_GT_PLAIN 0 /* VALU dot4, one query tile per workgroup */
+#define NBPG_GT_MFMA 1  /* int8 MFMA Gram tiles, exact int32 */
+
+/* (id, dist) record of the reference (query_params.h:97-104), 8 bytes */
+typedef struct {
+    uint32_t id;
+    float dist;
+} nbpg_iddist;
+
+/* device-resident index: vectors + adjacency + per-row norms.
+ * Replaces PointRange (point_range.h:33-325) and Graph (graph.h:119-400). */
+typedef struct nbpg_index nbpg_index;
+
+/* QueryParams (query_params.h:12-42), pure-quantized fields */
+typedef struct {
+    uint32_t k;            /* results per query; k <= L */
+    double cut;            /* frontier cut factor, 1.35 in the sweep (check_nn_recall.h:213) */
+    uint32_t L;            /* beamSize */
+    uint64_t limit;        /* max expansions per query (|V| in the sweep) */
+    uint32_t degree_limit; /* neighbours expanded per node (R in the sweep) */
+} nbpg_query_params;
+
+/* FAST-mode launch shape (the autotuner's knobs) */
+typedef struct {
+    uint32_t waves_per_group; /* queries per workgroup: 1..16 */
+    uint32_t expand_width;    /* beam entries expanded per step: 1..4 */
+    uint32_t filter_log2;     /* LDS visited-filter slots = 1 << filter_log2 (8..14) */
+    uint32_t groups;          /* persistent workgroups; 0 = derive from occupancy */
+} nbpg_fast_config;
+
+/* build_vamana's flags (Vamana.cpp:14-85, nsw.h:547-551) */
+typedef struct {
+    uint32_t R, L, VL, top_degree, num_split, num_passes;
+    uint64_t limit; /* 0 = n */
+    double min_alpha, max_alpha, min_cos_angle, max_cos_angle;
+    int32_t dynamic_prune, rand_sp, repair_isolate;
+    uint64_t sp_seed; /* seed replacing randomInt's random_device (utilities.h:55-63) */
+} nbpg_build_params;
+
+/* ---------------------------------------------------------------- */
+const char *nbpg_last_error(void);
+/* selects the device and caches its properties; call once per process */
+int nbpg_init(int device);
+int nbpg_device_info(int *cu_count, int *lds_per_group, int *l2_bytes);
+
+/* ---- index lifecycle ----
+ * stands in for load_point_range / PointRange ctor (point_range.h:17-57)
+ * and Graph(max_degree, n) / Graph(file) (graph.h:163-221). */
+int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, uint32_t R);
+int nbpg_index_destroy(nbpg_index *idx);
+/* upload rows [first, first+count) from host memory with the given stride
+ * (the reference's 64-B padded rows, point_range.h:51-56, or packed) */
+int nbpg_index_set_points(nbpg_index *idx, const void *host_rows, uint64_t host_stride,
+                          uint64_t first, uint64_t count);
+/* adjacency in the reference's in-memory layout: row i = [deg, id...],
+ * R+1 u32 per row (graph.h:142-161); host memory */
+int nbpg_index_set_graph(nbpg_index *idx, const uint32_t *host_rows, uint64_t first,
+                         uint64_t count);
+int nbpg_index_get_graph(const nbpg_index *idx, uint32_t *host_rows, uint64_t first,
+                         uint64_t count);
+/* device views for callers that stage their own queries */
+int nbpg_index_info(const nbpg_index *idx, int *dtype, uint64_t *n, uint32_t *dims, uint32_t *R,
+                    uint32_t *row_stride);
+
+/* ---- distances: Point::distance / euclidean_distance (point.h:35-120, 303-314)
+ * out[i] = dist(query, row ids[i]) for i < n; d_query holds `dims` bytes. */
+int nbpg_l2_batch(const nbpg_index *idx, const void *d_query, const uint32_t *d_ids, uint64_t n,
+                  float *d_out, void *stream);
+/* row-pair form over caller buffers: out[i] = dist(a_i, b_i), rows packed */
+int nbpg_l2_pairs(int dtype, const void *d_a, const void *d_b, uint64_t n, uint32_t dims,
+                  float *d_out, void *stream);
+
+/* ---- search: qsearchAll (beamSearch.h:257-303) with get_sp = start points
+ * (check_nn_recall.h:282-297).  d_queries: nq rows of `dims` bytes,
+ * query_stride bytes apart.  Outputs nq*k ids / dists (id 0xFFFFFFFF past
+ * the frontier), optional stats nq*2 = {visited, dist_cmps} as QueryStatic
+ * (query_params.h:47-94). */
+int nbpg_search(const nbpg_index *idx, const void *d_queries, uint64_t nq, uint64_t query_stride,
+                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
+                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                uint32_t *d_out_stats, void *stream);
+/* FAST mode with per-query beam width and k (uneven batches); d_L, d_k are
+ * nq u32 each on the device, outputs have stride out_k per query */
+int nbpg_search_uneven(const nbpg_index *idx, const void *d_queries, uint64_t nq,
+                       uint64_t query_stride, const uint32_t *d_L, const uint32_t *d_k,
+                       uint32_t max_L, uint32_t out_k, uint32_t entry,
+                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                       uint32_t *d_out_stats, void *stream);
+
+/* ---- RobustPrune: Prune::sort_candidates + Prune::robustPrune
+ * (prune.h:49-87, 158-209) for a batch of nodes.  Node t's candidates are
+ * d_cand[d_offsets[t] .. d_offsets[t+1]) with dist = d(node, cand); they are
+ * deduped, sorted (dist, id), the node itself dropped, then pruned greedily
+ * with alpha (d_alpha[t] if non-NULL, else alpha) and cos_angle.
+ * Output: up to R ids/dists per node at stride R, count in d_out_counts. */
+int nbpg_robust_prune_batch(const nbpg_index *idx, const uint32_t *d_nodes, uint64_t n_nodes,
+                            const nbpg_iddist *d_cand, const uint64_t *d_offsets, uint32_t R,
+                            const float *d_alpha, double alpha, double cos_angle,
+                            uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_counts,
+                            void *stream);
+
+/* ---- brute-force ground truth: compute_groundtruth (CompGT.cpp:29-134)
+ * exact (dist, id) top-k per query, rows packed (dims bytes each) */
+int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries, uint64_t nq,
+                uint32_t dims, uint32_t k, int mode, uint32_t *d_out_ids, float *d_out_dists,
+                void *stream);
+
+/* ---- graph build: build_vamana (nsw.h:547-663) on the device; the
+ * index's points must be set; the graph is (re)built in place (set a graph
+ * first to continue from it, Vamana.cpp:174-177). */
+int nbpg_build(nbpg_index *idx, const nbpg_build_params *bp, uint64_t *out_isolated);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

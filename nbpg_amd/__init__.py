@@ -1,0 +1,256 @@
+"""nbpg_amd -- MI355X-native NBPG hot path (beam search, RobustPrune, CompGT).
+
+Python front end over the C-ABI in include/nbpg.h.  The compute lives in
+nbpg_amd/libnbpg_amd.so (hand-written gfx950 HIP); this module only moves
+buffers: device memory comes from torch tensors on the current HIP device.
+There is no CPU fallback: every entry point raises when the library is
+missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbpg_amd.so")
+
+U8, I8, U4X2, I4X2 = 0, 1, 2, 3
+SEARCH_REF, SEARCH_FAST = 0, 1
+GT_PLAIN, GT_MFMA = 0, 1
+SENT = 0xFFFFFFFF
+DTYPE_OF_EXT = {".u8bin": U8, ".i8bin": I8, ".u4bin": U4X2, ".i4bin": I4X2}
+
+_lib = None
+
+
+class NbpgError(RuntimeError):
+    pass
+
+
+class QueryParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("cut", C.c_double), ("L", C.c_uint32),
+                ("limit", C.c_uint64), ("degree_limit", C.c_uint32)]
+
+
+class FastConfig(C.Structure):
+    _fields_ = [("waves_per_group", C.c_uint32), ("expand_width", C.c_uint32),
+                ("filter_log2", C.c_uint32), ("groups", C.c_uint32)]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("R", C.c_uint32), ("L", C.c_uint32), ("VL", C.c_uint32),
+                ("top_degree", C.c_uint32), ("num_split", C.c_uint32), ("num_passes", C.c_uint32),
+                ("limit", C.c_uint64), ("min_alpha", C.c_double), ("max_alpha", C.c_double),
+                ("min_cos_angle", C.c_double), ("max_cos_angle", C.c_double),
+                ("dynamic_prune", C.c_int32), ("rand_sp", C.c_int32),
+                ("repair_isolate", C.c_int32), ("sp_seed", C.c_uint64)]
+
+
+VP = C.c_void_p
+U32, U64, I32, DBL = C.c_uint32, C.c_uint64, C.c_int, C.c_double
+
+_SIGS = {
+    "nbpg_last_error": (C.c_char_p, []),
+    "nbpg_init": (I32, [I32]),
+    "nbpg_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "nbpg_index_create": (I32, [C.POINTER(VP), I32, U64, U32, U32]),
+    "nbpg_index_destroy": (I32, [VP]),
+    "nbpg_index_set_points": (I32, [VP, VP, U64, U64, U64]),
+    "nbpg_index_set_graph": (I32, [VP, VP, U64, U64]),
+    "nbpg_index_get_graph": (I32, [VP, VP, U64, U64]),
+    "nbpg_index_info": (I32, [VP, C.POINTER(I32), C.POINTER(U64), C.POINTER(U32), C.POINTER(U32),
+                              C.POINTER(U32)]),
+    "nbpg_l2_batch": (I32, [VP, VP, VP, U64, VP, VP]),
+    "nbpg_l2_pairs": (I32, [I32, VP, VP, U64, U32, VP, VP]),
+    "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
+                          C.POINTER(FastConfig), VP, VP, VP, VP]),
+    "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, U32, C.POINTER(FastConfig),
+                                 VP, VP, VP, VP]),
+    "nbpg_robust_prune_batch": (I32, [VP, VP, U64, VP, VP, U32, VP, DBL, DBL, VP, VP, VP, VP]),
+    "nbpg_compgt": (I32, [I32, VP, U64, VP, U64, U32, U32, I32, VP, VP, VP]),
+    "nbpg_build": (I32, [VP, C.POINTER(BuildParams), C.POINTER(U64)]),
+}
+
+
+def lib():
+    """Load libnbpg_amd.so; raises NbpgError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NbpgError("libnbpg_amd.so is not built (run __graft_entry__.build() or python nbpg_amd/make_lib.py)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().nbpg_last_error().decode(errors="replace")
+        raise NbpgError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise NbpgError("no HIP device visible")
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def init(device=0):
+    _check(lib().nbpg_init(device), "nbpg_init")
+
+
+def device_info():
+    cu, lds, l2 = I32(), I32(), I32()
+    _check(lib().nbpg_device_info(C.byref(cu), C.byref(lds), C.byref(l2)), "nbpg_device_info")
+    return {"cu": cu.value, "lds_per_group": lds.value, "l2": l2.value}
+
+
+class Index:
+    """Device-resident vectors + adjacency (see include/nbpg.h)."""
+
+    def __init__(self, dtype, n, dims, R):
+        _torch()
+        h = VP()
+        _check(lib().nbpg_index_create(C.byref(h), dtype, n, dims, R), "nbpg_index_create")
+        self.h, self.dtype, self.n, self.dims, self.R = h, dtype, n, dims, R
+
+    def close(self):
+        if self.h:
+            lib().nbpg_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_points(self, rows, first=0):
+        a = np.ascontiguousarray(rows)
+        assert a.ndim == 2 and a.shape[1] * a.itemsize >= self.dims
+        _check(lib().nbpg_index_set_points(self.h, a.ctypes.data, a.shape[1] * a.itemsize, first,
+                                           a.shape[0]), "nbpg_index_set_points")
+
+    def set_graph(self, rows, first=0):
+        a = np.ascontiguousarray(rows, dtype=np.uint32)
+        assert a.ndim == 2 and a.shape[1] == self.R + 1
+        _check(lib().nbpg_index_set_graph(self.h, a.ctypes.data, first, a.shape[0]),
+               "nbpg_index_set_graph")
+
+    def get_graph(self, first=0, count=None):
+        count = self.n - first if count is None else count
+        a = np.zeros((count, self.R + 1), np.uint32)
+        _check(lib().nbpg_index_get_graph(self.h, a.ctypes.data, first, count),
+               "nbpg_index_get_graph")
+        return a
+
+    @property
+    def row_stride(self):
+        s = U32()
+        _check(lib().nbpg_index_info(self.h, None, None, None, None, C.byref(s)), "nbpg_index_info")
+        return s.value
+
+
+def to_device(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def l2_batch(index, q_dev, ids_dev):
+    torch = _torch()
+    out = torch.empty(ids_dev.numel(), dtype=torch.float32, device="cuda")
+    _check(lib().nbpg_l2_batch(index.h, _ptr(q_dev), _ptr(ids_dev), ids_dev.numel(), _ptr(out),
+                               _stream()), "nbpg_l2_batch")
+    return out
+
+
+def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None,
+           entry=0, start_points=None, fast=None, stats=True, out=None):
+    """Beam search for every row of q_dev (uint8 tensor, nq x stride bytes)."""
+    torch = _torch()
+    nq = q_dev.shape[0]
+    qstride = q_dev.shape[1] * q_dev.element_size()
+    qp = QueryParams(k, cut, L, index.n if limit is None else limit,
+                     index.R if degree_limit is None else degree_limit)
+    sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
+    if out is None:
+        ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        st = torch.empty((nq, 2), dtype=torch.int32, device="cuda") if stats else None
+    else:
+        ids, d, st = out
+    fc = FastConfig(*fast) if fast is not None else None
+    _check(lib().nbpg_search(index.h, _ptr(q_dev), nq, qstride, sp.ctypes.data, len(sp),
+                             C.byref(qp), mode, C.byref(fc) if fc is not None else None,
+                             _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search")
+    return ids, d, st
+
+
+def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None):
+    torch = _torch()
+    nq = q_dev.shape[0]
+    qstride = q_dev.shape[1] * q_dev.element_size()
+    ids = torch.empty((nq, out_k), dtype=torch.int32, device="cuda")
+    d = torch.empty((nq, out_k), dtype=torch.float32, device="cuda")
+    st = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+    fc = FastConfig(*fast) if fast is not None else None
+    _check(lib().nbpg_search_uneven(index.h, _ptr(q_dev), nq, qstride, _ptr(L_dev), _ptr(k_dev),
+                                    max_L, out_k, entry, C.byref(fc) if fc is not None else None,
+                                    _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search_uneven")
+    return ids, d, st
+
+
+def robust_prune_batch(index, nodes_dev, cand_dev, offsets_dev, R, alpha=1.2, cos_angle=0.0,
+                       alphas_dev=None):
+    """cand_dev: int64 tensor of packed (id u32, dist f32) records."""
+    torch = _torch()
+    n = nodes_dev.numel()
+    ids = torch.empty((n, R), dtype=torch.int32, device="cuda")
+    d = torch.empty((n, R), dtype=torch.float32, device="cuda")
+    cnt = torch.empty(n, dtype=torch.int32, device="cuda")
+    _check(lib().nbpg_robust_prune_batch(index.h, _ptr(nodes_dev), n, _ptr(cand_dev),
+                                         _ptr(offsets_dev), R, _ptr(alphas_dev), alpha, cos_angle,
+                                         _ptr(ids), _ptr(d), _ptr(cnt), _stream()),
+           "nbpg_robust_prune_batch")
+    return ids, d, cnt
+
+
+def compgt(base_dev, q_dev, k, dtype=U8, mode=GT_PLAIN):
+    torch = _torch()
+    nq = q_dev.shape[0]
+    ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+    _check(lib().nbpg_compgt(dtype, _ptr(base_dev), base_dev.shape[0], _ptr(q_dev), nq,
+                             base_dev.shape[1], k, mode, _ptr(ids), _ptr(d), _stream()),
+           "nbpg_compgt")
+    return ids, d
+
+
+def pack_cands(ids, dists):
+    """(id u32, dist f32) records as one int64 array (little-endian)."""
+    rec = np.zeros(len(ids), dtype=[("id", np.uint32), ("dist", np.float32)])
+    rec["id"] = ids
+    rec["dist"] = dists
+    return rec.view(np.int64)
+
+
+def build(index, R, L, alpha=1.2, passes=1, num_split=0, VL=140, min_alpha=1.0,
+          dynamic_prune=False, rand_sp=False, repair=True, top_degree=0, sp_seed=1, limit=0):
+    bp = BuildParams(R, L, VL, top_degree, num_split, passes, limit, min_alpha, alpha, 0.0, 0.0,
+                     int(dynamic_prune), int(rand_sp), int(repair), sp_seed)
+    iso = U64(0)
+    _check(lib().nbpg_build(index.h, C.byref(bp), C.byref(iso)), "nbpg_build")
+    return iso.value

@@ -1,0 +1,165 @@
+// common.h -- device primitives shared by the gfx950 kernels.
+//
+// Distances are exact int32: |x-q|^2 = |x|^2 + |q|^2 - 2<x,q>, with the
+// per-row norm precomputed once at upload and the dot product on the packed
+// integer dot instructions (v_dot4_u32_u8, v_dot4_i32_i8, v_dot8_u32_u4).
+// This equals the reference's sum of squared differences
+// (point.h:35-44, 56-65, 77-90, 107-120) bit for bit, because every term is
+// an integer and the final value fits in int32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nbpg.h"
+
+#define NBPG_SENT 0xFFFFFFFFu
+#define WAVE 64
+
+namespace nbpg {
+
+// ---------------------------------------------------------------- types
+template <int DT> struct dot_traits;
+
+template <> struct dot_traits<NBPG_U8> {
+    static __device__ __forceinline__ uint32_t dot(uint32_t a, uint32_t b, uint32_t acc)
+    {
+        return __builtin_amdgcn_udot4(a, b, acc, false);
+    }
+    static constexpr uint32_t scale = 1;
+};
+template <> struct dot_traits<NBPG_I8> {
+    static __device__ __forceinline__ uint32_t dot(uint32_t a, uint32_t b, uint32_t acc)
+    {
+        return (uint32_t)__builtin_amdgcn_sdot4((int)a, (int)b, (int)acc, false);
+    }
+    static constexpr uint32_t scale = 1;
+};
+// both packed 4-bit types: the reference splits a byte into b & 0xF0 and
+// (b << 4) & 0xFF, i.e. 16 x (unsigned nibble); squared differences scale by 256
+template <> struct dot_traits<NBPG_U4X2> {
+    static __device__ __forceinline__ uint32_t dot(uint32_t a, uint32_t b, uint32_t acc)
+    {
+        return __builtin_amdgcn_udot8(a, b, acc, false);
+    }
+    static constexpr uint32_t scale = 256;
+};
+template <> struct dot_traits<NBPG_I4X2> : dot_traits<NBPG_U4X2> {};
+
+// <x, q> over NCH 16-byte chunks; q held in registers
+template <int DT, int NCH>
+__device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row, const uint4 (&q)[NCH])
+{
+    uint4 x[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        x[c] = row[c];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        acc = dot_traits<DT>::dot(x[c].x, q[c].x, acc);
+        acc = dot_traits<DT>::dot(x[c].y, q[c].y, acc);
+        acc = dot_traits<DT>::dot(x[c].z, q[c].z, acc);
+        acc = dot_traits<DT>::dot(x[c].w, q[c].w, acc);
+    }
+    return acc;
+}
+
+// exact reference distance as float: (float)(int32) sum of squares
+template <int DT>
+__device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t dot)
+{
+    uint32_t s = nx + nq - 2u * dot; // wraps back into range: true value < 2^31
+    return (float)(int32_t)(s * dot_traits<DT>::scale);
+}
+
+template <int DT, int NCH>
+__device__ __forceinline__ uint32_t norm_regs(const uint4 (&q)[NCH])
+{
+    uint32_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        acc = dot_traits<DT>::dot(q[c].x, q[c].x, acc);
+        acc = dot_traits<DT>::dot(q[c].y, q[c].y, acc);
+        acc = dot_traits<DT>::dot(q[c].z, q[c].z, acc);
+        acc = dot_traits<DT>::dot(q[c].w, q[c].w, acc);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- hashing
+// splitmix64 finaliser (utilities.h:31-37); the visited-filter slot hash
+__device__ __forceinline__ uint64_t hash64_2(uint64_t x)
+{
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// ---------------------------------------------------------------- keys
+// (dist, id) records ordered as the reference's id_dist_less
+// (query_params.h:115-119).  Distances are non-negative floats, whose bit
+// patterns order like the values, so (bits(dist) << 32 | id) orders as
+// (dist asc, id asc) under plain u64 comparison.
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t id)
+{
+    return ((uint64_t)__float_as_uint(d) << 32) | id;
+}
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)k; }
+__device__ __forceinline__ float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+// ---------------------------------------------------------------- wave
+__device__ __forceinline__ uint32_t lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// number of set bits of mask in lanes below the calling lane
+__device__ __forceinline__ uint32_t prefix_count(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, (uint32_t)__shfl_xor(v, o));
+    return v;
+}
+// compiler + memory ordering point for lanes of one wavefront (LDS traffic
+// of a wave is processed in order; this keeps the compiler from reordering)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <typename T> __device__ __forceinline__ T bcast(T v, int lane) { return __shfl(v, lane); }
+
+// #entries e of sorted a[0:n) with e < v (lower bound), per-lane search
+__device__ __forceinline__ uint32_t count_less(const uint64_t *a, uint32_t n, uint64_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ bool contains(const uint64_t *a, uint32_t n, uint64_t v)
+{
+    uint32_t p = count_less(a, n, v);
+    return p < n && a[p] == v;
+}
+
+} // namespace nbpg

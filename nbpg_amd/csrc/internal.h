@@ -1,0 +1,94 @@
+// internal.h -- host-side structures shared by the C-ABI translation units.
+#pragma once
+#include <stdarg.h>
+#include <string.h>
+#include <stddef.h>
+
+#include <type_traits>
+#include <utility>
+
+#include "common.h"
+
+// Device-resident index.  Layout in HBM:
+//   vecs  : n rows x stride bytes, stride = 16 * nch (zero padded), so one
+//           row is nch dwordx4 loads and any row of <= 128 B sits in one
+//           128-B line when the stride is 128
+//   norms : n x u32, |row|^2 in the type's dot units
+//   nbr   : n x R u32 neighbour ids, unused slots = 0xFFFFFFFF (no degree
+//           read on the search path: a row is one contiguous R*4-B gather)
+//   deg   : n x u32 degrees (build and graph download)
+struct nbpg_index {
+    int dtype;
+    uint64_t n;
+    uint32_t dims;
+    uint32_t R;
+    uint32_t nch;
+    uint32_t stride;
+    uint8_t *vecs;
+    uint32_t *norms;
+    uint32_t *nbr;
+    uint32_t *deg;
+    void *ws; // grow-only workspace for search/prune scratch
+    size_t ws_bytes;
+    uint32_t *counter; // small device buffer: work queue counters and flags
+};
+
+namespace nbpg {
+
+int set_error(int code, const char *fmt, ...);
+
+#define HIP_TRY(x)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return nbpg::set_error(NBPG_EHIP, "%s failed: %s", #x, hipGetErrorString(e_));        \
+    } while (0)
+
+#define NBPG_TRY(x)                                                                                \
+    do {                                                                                           \
+        int r_ = (x);                                                                              \
+        if (r_ != NBPG_OK)                                                                         \
+            return r_;                                                                             \
+    } while (0)
+
+struct DevInfo {
+    int device = -1;
+    int cu = 0;
+    int lds_max = 0;
+    int l2 = 0;
+    bool inited = false;
+};
+extern DevInfo g_dev;
+
+int ensure_init();
+int ensure_ws(nbpg_index *idx, size_t bytes);
+// 16-byte chunks per row for `bytes` of payload: one of 2, 4, 6, 8, 16
+uint32_t nch_for_bytes(uint32_t bytes);
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// calls f(std::integral_constant<int, DT>, std::integral_constant<int, NCH>)
+// for the (dtype, nch) pair; the two packed 4-bit types share kernels
+template <typename F> int dispatch_dt_nch(int dtype, uint32_t nch, F &&f)
+{
+    using std::integral_constant;
+#define NBPG_NCH_CASES(DTV)                                                                        \
+    switch (nch) {                                                                                 \
+    case 2: f(integral_constant<int, DTV>(), integral_constant<int, 2>()); return NBPG_OK;         \
+    case 4: f(integral_constant<int, DTV>(), integral_constant<int, 4>()); return NBPG_OK;         \
+    case 6: f(integral_constant<int, DTV>(), integral_constant<int, 6>()); return NBPG_OK;         \
+    case 8: f(integral_constant<int, DTV>(), integral_constant<int, 8>()); return NBPG_OK;         \
+    case 16: f(integral_constant<int, DTV>(), integral_constant<int, 16>()); return NBPG_OK;       \
+    default: break;                                                                                \
+    }
+    switch (dtype) {
+    case NBPG_U8: NBPG_NCH_CASES(NBPG_U8) break;
+    case NBPG_I8: NBPG_NCH_CASES(NBPG_I8) break;
+    case NBPG_U4X2:
+    case NBPG_I4X2: NBPG_NCH_CASES(NBPG_U4X2) break;
+    default: break;
+    }
+#undef NBPG_NCH_CASES
+    return set_error(NBPG_EUNSUP, "unsupported dtype %d / row chunks %u", dtype, nch);
+}
+
+} // namespace nbpg

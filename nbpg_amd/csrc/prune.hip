@@ -1,0 +1,252 @@
+// prune.hip -- batched RobustPrune: Prune::sort_candidates (prune.h:49-74)
+// followed by Prune::limitedSimilarityPrune (prune.h:158-209), one
+// wavefront per node.
+//
+// Candidates are keyed (dist, id) with the centre node keyed at FLT_MAX, as
+// the reference's biased_less does with dir_bias 0; a bitonic sort in LDS
+// orders them, adjacent equal ids collapse (dedupe_by_index, prune.h:251-260),
+// trailing centre entries are dropped (prune.h:72-73).  The greedy scan then
+// takes candidates in order and, per selected p*, computes d(p*, p') for
+// every later live candidate on 64 lanes at once, pruning when
+// alpha * d(p*, p') <= d(p, p') in double (prune.h:227-230) or, when
+// cos_angle > 1e-5, when the angle test fires (prune.h:232-238).
+#include <float.h>
+
+#include <algorithm>
+
+#include "internal.h"
+
+namespace nbpg {
+
+#define PRUNE_MAXC 1024
+#define PRUNE_WAVES 4
+
+struct PruneArgs {
+    const uint8_t *vecs;
+    const uint32_t *norms;
+    uint64_t n;
+    uint32_t stride;
+    const uint32_t *nodes;
+    uint64_t n_nodes;
+    const nbpg_iddist *cand;
+    const uint64_t *offsets;
+    uint32_t R;
+    const float *alphas;
+    double alpha, cos_angle;
+    uint32_t *out_ids;
+    float *out_dists;
+    uint32_t *out_counts;
+    uint32_t *out_comps; // optional distance counts per node
+    uint32_t *flag;      // bit 0: a node had > PRUNE_MAXC candidates; bit 1: bad id
+};
+
+// ascending bitonic sort of a[0:n2) (n2 a power of two) by one wavefront
+__device__ __forceinline__ void wave_bitonic(uint64_t *a, uint32_t n2, uint32_t lane)
+{
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < n2; i += WAVE) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const uint64_t x = a[i], y = a[p];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[p] = x;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+__device__ __forceinline__ float cosine_at_p(float d_ps, float d_pp, float d_sp)
+{
+    const float b = sqrtf(d_pp), c = sqrtf(d_ps);
+    if (b == 0.f || c == 0.f)
+        return 1.f;
+    return (d_pp + d_ps - d_sp) / (2.f * b * c);
+}
+
+template <int DT, int NCH>
+__global__ __launch_bounds__(64 * PRUNE_WAVES) void prune_kernel(PruneArgs a)
+{
+    __shared__ uint64_t keys_all[PRUNE_WAVES][PRUNE_MAXC];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / WAVE;
+    uint64_t *keys = keys_all[wv];
+    const uint64_t t = (uint64_t)blockIdx.x * PRUNE_WAVES + wv;
+    if (t >= a.n_nodes)
+        return;
+    const uint32_t p = a.nodes[t];
+    const uint64_t b = a.offsets[t], e = a.offsets[t + 1];
+    uint32_t *oid = a.out_ids + t * a.R;
+    float *od = a.out_dists ? a.out_dists + t * a.R : nullptr;
+    if (e < b || e - b > PRUNE_MAXC || p >= a.n) {
+        if (lane == 0) {
+            atomicOr(a.flag, p >= a.n ? 2u : 1u);
+            a.out_counts[t] = 0;
+        }
+        return;
+    }
+    const uint32_t n = (uint32_t)(e - b);
+    uint32_t n2 = 1;
+    while (n2 < n)
+        n2 <<= 1;
+    bool bad = false;
+    for (uint32_t i = lane; i < n2; i += WAVE) {
+        uint64_t k = ~0ull;
+        if (i < n) {
+            const nbpg_iddist c = a.cand[b + i];
+            bad |= c.id >= a.n;
+            k = make_key(c.id == p ? FLT_MAX : c.dist, c.id);
+        }
+        keys[i] = k;
+    }
+    if (ballot(bad)) {
+        if (lane == 0) {
+            atomicOr(a.flag, 2u);
+            a.out_counts[t] = 0;
+        }
+        return;
+    }
+    wave_sync();
+    wave_bitonic(keys, n2, lane);
+    // collapse equal ids (equal ids carry equal keys once sorted)
+    uint32_t m = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
+        const uint32_t i = i0 + lane;
+        const uint64_t v = i < n ? keys[i] : 0;
+        const bool keep = i < n && (i == 0 || key_id(keys[i - 1]) != key_id(v));
+        const uint64_t km = ballot(keep);
+        wave_sync();
+        if (keep)
+            keys[m + prefix_count(km)] = v;
+        m += (uint32_t)__popcll(km);
+        wave_sync();
+    }
+    // drop trailing centre entries
+    while (m > 0 && key_id(keys[m - 1]) == p)
+        --m;
+    const double alpha = a.alphas ? (double)a.alphas[t] : a.alpha;
+    const double cosa = a.cos_angle;
+    const double eps = 1e-5;
+    uint32_t sel = 0, comps = 0;
+    if (alpha <= eps && cosa <= eps) { // prune.h:164-169
+        sel = min(m, a.R);
+        for (uint32_t i = lane; i < sel; i += WAVE) {
+            oid[i] = key_id(keys[i]);
+            if (od)
+                od[i] = key_dist(keys[i]);
+        }
+    } else {
+        uint32_t scan = 0;
+        while (sel < a.R && scan < m) {
+            const uint64_t ks = keys[scan];
+            const uint32_t ps = key_id(ks);
+            ++scan;
+            if (ps == p || ps == NBPG_SENT)
+                continue;
+            const float d_ps = key_dist(ks);
+            if (lane == 0) {
+                oid[sel] = ps;
+                if (od)
+                    od[sel] = d_ps;
+            }
+            ++sel;
+            uint4 sv[NCH];
+            const uint4 *sr = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ps * a.stride);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                sv[c] = sr[c];
+            const uint32_t ns = a.norms[ps];
+            for (uint32_t i0 = scan; i0 < m; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t kv = i < m ? keys[i] : ~0ull;
+                const uint32_t pp = key_id(kv);
+                const bool live = i < m && pp != NBPG_SENT;
+                bool prune = false;
+                if (live) {
+                    const uint4 *r =
+                        reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)pp * a.stride);
+                    const float d_sp = dist_from<DT>(a.norms[pp], ns, dot_row<DT, NCH>(r, sv));
+                    const float d_pp = key_dist(kv);
+                    if (alpha > eps && alpha * (double)d_sp <= (double)d_pp)
+                        prune = true;
+                    else if (cosa > eps && (double)cosine_at_p(d_ps, d_pp, d_sp) > cosa)
+                        prune = true;
+                }
+                comps += (uint32_t)__popcll(ballot(live));
+                if (prune)
+                    keys[i] = (kv & 0xFFFFFFFF00000000ull) | NBPG_SENT;
+            }
+            wave_sync();
+        }
+    }
+    if (lane == 0) {
+        a.out_counts[t] = sel;
+        if (a.out_comps)
+            a.out_comps[t] = comps;
+    }
+}
+
+int launch_prune(const nbpg_index *ix, const PruneArgs &a, hipStream_t stream)
+{
+    const uint32_t blocks = (uint32_t)((a.n_nodes + PRUNE_WAVES - 1) / PRUNE_WAVES);
+    int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
+        hipLaunchKernelGGL((prune_kernel<DT.value, NCH.value>), dim3(blocks), dim3(64 * PRUNE_WAVES),
+                           0, stream, a);
+    });
+    NBPG_TRY(rc);
+    HIP_TRY(hipGetLastError());
+    return NBPG_OK;
+}
+
+} // namespace nbpg
+
+using namespace nbpg;
+
+extern "C" int nbpg_robust_prune_batch(const nbpg_index *ix, const uint32_t *d_nodes,
+                                       uint64_t n_nodes, const nbpg_iddist *d_cand,
+                                       const uint64_t *d_offsets, uint32_t R,
+                                       const float *d_alpha, double alpha, double cos_angle,
+                                       uint32_t *d_out_ids, float *d_out_dists,
+                                       uint32_t *d_out_counts, void *stream)
+{
+    if (!ix || (n_nodes && (!d_nodes || !d_offsets || !d_out_ids || !d_out_counts)))
+        return set_error(NBPG_EINVAL, "null argument");
+    if (R == 0)
+        return set_error(NBPG_EINVAL, "R = 0");
+    if (n_nodes == 0)
+        return NBPG_OK;
+    PruneArgs a;
+    memset(&a, 0, sizeof(a));
+    a.vecs = ix->vecs;
+    a.norms = ix->norms;
+    a.n = ix->n;
+    a.stride = ix->stride;
+    a.nodes = d_nodes;
+    a.n_nodes = n_nodes;
+    a.cand = d_cand;
+    a.offsets = d_offsets;
+    a.R = R;
+    a.alphas = d_alpha;
+    a.alpha = alpha;
+    a.cos_angle = cos_angle;
+    a.out_ids = d_out_ids;
+    a.out_dists = d_out_dists;
+    a.out_counts = d_out_counts;
+    a.flag = ix->counter + 40;
+    hipStream_t s = as_stream(stream);
+    HIP_TRY(hipMemsetAsync(a.flag, 0, 4, s));
+    NBPG_TRY(launch_prune(ix, a, s));
+    uint32_t f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, a.flag, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (f & 2)
+        return set_error(NBPG_EINVAL, "node or candidate id >= n");
+    if (f & 1)
+        return set_error(NBPG_EUNSUP, "more than %d candidates for a node", PRUNE_MAXC);
+    return NBPG_OK;
+}

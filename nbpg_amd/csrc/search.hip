@@ -1,0 +1,745 @@
+// search.hip -- greedy beam search over the device-resident graph.
+//
+// NBPG_SEARCH_REF: one 64-lane workgroup per query replaying
+//   filtered_beam_search (beamSearch.h:43-171) step for step: the lossy
+//   direct-mapped visited filter with overwrite-on-miss in neighbour order
+//   (query_params.h:223-230, size query_params.h:243-247), the sorted
+//   visited list capped at 8L (beamSearch.h:95-117, 279), the delayed merge
+//   (beamSearch.h:119-125), sort+unique+set_union (128-138), the k-th cut
+//   (140-149) and the frontier \ visited rescan (154-160).  Result IDs and
+//   the {visited, dist_cmps} counters equal the reference's.
+// NBPG_SEARCH_FAST: persistent kernel, one wavefront per query pulled from
+//   an atomic work counter; LDS-resident lossy visited filter and sorted
+//   beam; W beam entries expanded per step; candidates worse than the beam's
+//   tail are dropped before the merge.
+#include <algorithm>
+
+#include "internal.h"
+
+namespace nbpg {
+
+template <int NCH>
+__device__ __forceinline__ void load_query(const uint8_t *__restrict__ q, uint32_t qbytes,
+                                           uint4 (&qv)[NCH])
+{
+    const bool aligned = (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        if (aligned && (uint32_t)(c * 16 + 16) <= qbytes) {
+            qv[c] = reinterpret_cast<const uint4 *>(q)[c];
+        } else {
+            uint32_t w[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                uint32_t v = 0;
+                for (uint32_t b = 0; b < 4; ++b) {
+                    uint32_t off = c * 16 + t * 4 + b;
+                    if (off < qbytes)
+                        v |= (uint32_t)q[off] << (8 * b);
+                }
+                w[t] = v;
+            }
+            qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+struct SearchArgs {
+    const uint8_t *vecs;
+    const uint32_t *norms;
+    const uint32_t *nbr;
+    uint64_t n;
+    uint32_t stride, R;
+    const uint8_t *queries;
+    uint64_t qstride, nq;
+    uint32_t qbytes;
+    // reference mode
+    uint32_t sp[32];
+    uint32_t sp_num;
+    uint32_t k, L, dlim, maxv, H;
+    double cut;
+    uint64_t limit;
+    uint8_t *gscratch;
+    uint64_t state_bytes;
+    // fast mode
+    const uint32_t *Ls, *ks;
+    uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;
+    uint32_t *counter;
+    // outputs
+    uint32_t *out_ids;
+    float *out_dists;
+    uint32_t *out_stats;
+};
+
+// --------------------------------------------------------------------------
+// reference-exact mode
+// --------------------------------------------------------------------------
+
+struct RefLayout {
+    uint64_t fa, fb, unv, cand, cs, vis, total;
+};
+
+__host__ __device__ inline RefLayout ref_layout(uint32_t L, uint32_t R, uint32_t maxv, uint32_t H)
+{
+    RefLayout o;
+    uint64_t p = ((uint64_t)H * 4 + 15) & ~15ull;
+    const uint64_t fsz = 2ull * L + R, csz = (uint64_t)L + R + 64;
+    o.fa = p;
+    p += 8 * fsz;
+    o.fb = p;
+    p += 8 * fsz;
+    o.unv = p;
+    p += 8ull * L;
+    o.cand = p;
+    p += 8 * csz;
+    o.cs = p;
+    p += 8 * csz;
+    o.vis = p;
+    p += 8ull * maxv;
+    o.total = (p + 15) & ~15ull;
+    return o;
+}
+
+// dst[rank(src[i])] = src[i]; stable for equal keys; one wave
+__device__ __forceinline__ void wave_rank_sort(const uint64_t *src, uint32_t n, uint64_t *dst,
+                                               uint32_t lane)
+{
+    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
+        uint32_t i = i0 + lane;
+        uint64_t v = i < n ? src[i] : 0;
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            uint64_t u = src[j];
+            r += (u < v) || (u == v && j < i);
+        }
+        if (i < n)
+            dst[r] = v;
+    }
+}
+
+template <int DT, int NCH, bool LDS>
+__global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
+{
+    extern __shared__ __align__(16) uint8_t lds_state[];
+    uint8_t *st = LDS ? lds_state : a.gscratch + (uint64_t)blockIdx.x * a.state_bytes;
+    const uint32_t L = a.L, R = a.R, H = a.H, maxv = a.maxv;
+    const RefLayout lo = ref_layout(L, R, maxv, H);
+    uint32_t *hash = reinterpret_cast<uint32_t *>(st);
+    uint64_t *F = reinterpret_cast<uint64_t *>(st + lo.fa);
+    uint64_t *NF = reinterpret_cast<uint64_t *>(st + lo.fb);
+    uint64_t *unv = reinterpret_cast<uint64_t *>(st + lo.unv);
+    uint64_t *cand = reinterpret_cast<uint64_t *>(st + lo.cand);
+    uint64_t *cs = reinterpret_cast<uint64_t *>(st + lo.cs);
+    uint64_t *vis = reinterpret_cast<uint64_t *>(st + lo.vis);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lim = min(R, a.dlim);
+
+    for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
+        uint4 qv[NCH];
+        load_query<NCH>(a.queries + qi * a.qstride, a.qbytes, qv);
+        const uint32_t nqn = norm_regs<DT, NCH>(qv);
+        auto dist_of = [&](uint32_t id) -> float {
+            const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)id * a.stride);
+            return dist_from<DT>(a.norms[id], nqn, dot_row<DT, NCH>(r, qv));
+        };
+        uint64_t *Fp = F, *NFp = NF;
+        for (uint32_t i = lane; i < H; i += WAVE)
+            hash[i] = NBPG_SENT;
+        __syncthreads();
+        // init_starting_points (query_params.h:203-221): filter in order
+        if (lane == 0)
+            for (uint32_t s = 0; s < a.sp_num; ++s) {
+                uint32_t slot = (uint32_t)(hash64_2(a.sp[s]) & (H - 1));
+                hash[slot] = a.sp[s];
+            }
+        for (uint32_t s = lane; s < a.sp_num; s += WAVE)
+            cand[s] = make_key(dist_of(a.sp[s]), a.sp[s]);
+        __syncthreads();
+        wave_rank_sort(cand, a.sp_num, Fp, lane);
+        __syncthreads();
+        for (uint32_t s = lane; s < a.sp_num; s += WAVE)
+            unv[s] = Fp[s];
+        uint32_t fs = a.sp_num, remain = a.sp_num, offset = 0, cand_n = 0, vs = 0;
+        uint64_t nvis = 0, cmps = a.sp_num;
+        __syncthreads();
+
+        while (remain > offset && nvis < a.limit) {
+            const uint64_t cur = unv[offset];
+            const uint32_t cid = key_id(cur);
+            // 1) neighbour filter, then 2) distances (beamSearch.h:71-93)
+            for (uint32_t j0 = 0; j0 < lim; j0 += WAVE) {
+                const uint32_t j = j0 + lane;
+                const uint32_t av = j < lim ? a.nbr[(uint64_t)cid * R + j] : NBPG_SENT;
+                const bool valid = av != NBPG_SENT;
+                const uint64_t vm = ballot(valid);
+                const uint32_t slot = valid ? (uint32_t)(hash64_2(av) & (H - 1)) : 0u;
+                const uint32_t hold = valid ? hash[slot] : 0u;
+                // lanes sharing a slot act in lane order: a lane sees the id
+                // written by the nearest lower lane of its slot, if any
+                bool has_pred = false, is_last = true;
+                uint32_t pred_a = 0;
+                for (uint64_t m = vm; m; m &= m - 1) {
+                    const int jl = __builtin_ctzll(m);
+                    const uint32_t sj = __builtin_amdgcn_readlane(slot, jl);
+                    const uint32_t aj = __builtin_amdgcn_readlane(av, jl);
+                    if (valid && sj == slot) {
+                        if (jl < (int)lane) {
+                            has_pred = true;
+                            pred_a = aj;
+                        } else if (jl > (int)lane) {
+                            is_last = false;
+                        }
+                    }
+                }
+                const bool seen = has_pred ? (pred_a == av) : (hold == av);
+                __syncthreads();
+                if (valid && is_last)
+                    hash[slot] = av;
+                const bool fresh = valid && !seen;
+                const uint64_t um = ballot(fresh);
+                if (fresh)
+                    cand[cand_n + prefix_count(um)] = make_key(dist_of(av), av);
+                const uint32_t nf = (uint32_t)__popcll(um);
+                cand_n += nf;
+                cmps += nf;
+                __syncthreads();
+            }
+            // 3) sorted insertion of `cur` into visited (beamSearch.h:95-117)
+            uint32_t c = 0;
+            for (uint32_t i = lane; i < vs; i += WAVE)
+                c += vis[i] <= cur;
+            const uint32_t pos = wave_sum(c);
+            ++nvis;
+            const uint32_t nvs = vs < maxv ? vs + 1 : vs;
+            if (pos + 1 < nvs) {
+                for (int32_t top = (int32_t)nvs - 2; top >= (int32_t)pos; top -= WAVE) {
+                    const int32_t idx = top - (int32_t)lane;
+                    uint64_t v = 0;
+                    if (idx >= (int32_t)pos)
+                        v = vis[idx];
+                    __syncthreads();
+                    if (idx >= (int32_t)pos)
+                        vis[idx + 1] = v;
+                    __syncthreads();
+                }
+            }
+            if (pos < nvs && lane == 0)
+                vis[pos] = cur;
+            vs = nvs;
+            __syncthreads();
+            // 4) delayed merge (beamSearch.h:119-125)
+            if (cand_n == 0 ||
+                (a.limit >= 2ull * L && cand_n < L / 8 && offset + 1 < remain)) {
+                ++offset;
+                continue;
+            }
+            offset = 0;
+            // sort + unique by id (beamSearch.h:128-130)
+            wave_rank_sort(cand, cand_n, cs, lane);
+            __syncthreads();
+            uint32_t cu = 0;
+            for (uint32_t i0 = 0; i0 < cand_n; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t v = i < cand_n ? cs[i] : 0;
+                const bool keep = i < cand_n && (i == 0 || key_id(cs[i - 1]) != key_id(v));
+                const uint64_t km = ballot(keep);
+                if (keep)
+                    cand[cu + prefix_count(km)] = v;
+                cu += (uint32_t)__popcll(km);
+            }
+            __syncthreads();
+            // set_union(frontier, candidates) truncated to L (beamSearch.h:132-138)
+            uint32_t nb = 0;
+            for (uint32_t i0 = 0; i0 < cu; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t v = i < cu ? cand[i] : 0;
+                const bool keep = i < cu && !contains(Fp, fs, v);
+                const uint64_t km = ballot(keep);
+                if (keep)
+                    cs[nb + prefix_count(km)] = v;
+                nb += (uint32_t)__popcll(km);
+            }
+            __syncthreads();
+            for (uint32_t i = lane; i < fs; i += WAVE) {
+                const uint64_t v = Fp[i];
+                const uint32_t p = i + count_less(cs, nb, v);
+                if (p < L)
+                    NFp[p] = v;
+            }
+            for (uint32_t j = lane; j < nb; j += WAVE) {
+                const uint64_t v = cs[j];
+                const uint32_t p = j + count_less(Fp, fs, v);
+                if (p < L)
+                    NFp[p] = v;
+            }
+            uint32_t nfs = min(L, fs + nb);
+            __syncthreads();
+            // cut at cut * d(k-th) (beamSearch.h:140-149)
+            if (a.k > 0 && nfs > a.k) {
+                const float thr = (float)(a.cut * (double)key_dist(NFp[a.k]));
+                const uint64_t tk = make_key(thr, 0u);
+                uint32_t cc = 0;
+                for (uint32_t i = lane; i < nfs; i += WAVE)
+                    cc += NFp[i] <= tk;
+                const uint32_t ub = wave_sum(cc);
+                nfs = max(ub, fs);
+            }
+            {
+                uint64_t *t = Fp;
+                Fp = NFp;
+                NFp = t;
+            }
+            fs = nfs;
+            cand_n = 0;
+            // unvisited = frontier[0:min(fs,L)] \ visited (beamSearch.h:155-160)
+            const uint32_t fl = min(fs, L);
+            uint32_t r = 0;
+            for (uint32_t i0 = 0; i0 < fl; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t v = i < fl ? Fp[i] : 0;
+                const bool keep = i < fl && !contains(vis, vs, v);
+                const uint64_t km = ballot(keep);
+                if (keep)
+                    unv[r + prefix_count(km)] = v;
+                r += (uint32_t)__popcll(km);
+            }
+            remain = r;
+            __syncthreads();
+        }
+        // frontier[0:k] (beamSearch.h:249-254)
+        for (uint32_t j = lane; j < a.k; j += WAVE) {
+            const uint64_t o = qi * a.k + j;
+            if (j < fs) {
+                a.out_ids[o] = key_id(Fp[j]);
+                if (a.out_dists)
+                    a.out_dists[o] = key_dist(Fp[j]);
+            } else {
+                a.out_ids[o] = NBPG_SENT;
+                if (a.out_dists)
+                    a.out_dists[o] = __int_as_float(0x7f800000);
+            }
+        }
+        if (a.out_stats && lane == 0) {
+            a.out_stats[2 * qi] = (uint32_t)nvis;
+            a.out_stats[2 * qi + 1] = (uint32_t)cmps;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// fast mode
+// --------------------------------------------------------------------------
+
+#define FAST_FLAG 0x80000000ull // bit 31 of the id word: entry expanded
+
+__device__ __forceinline__ uint64_t unflag(uint64_t k) { return k & ~FAST_FLAG; }
+
+__device__ __forceinline__ uint32_t count_less_unflag(const uint64_t *a, uint32_t n, uint64_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (unflag(a[mid]) < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
+{
+    return (id * 0x9E3779B1u) >> (32 - flog2);
+}
+
+__host__ __device__ inline uint32_t fast_wave_bytes(uint32_t Lmax, uint32_t ipl, uint32_t flog2)
+{
+    // filter | beam x2 | cand x2 | selected ids
+    uint32_t b = (4u << flog2) + 16u * Lmax + 2u * 8u * 64u * ipl + 16u;
+    return (b + 15u) & ~15u;
+}
+
+template <int DT, int NCH, int IPL>
+__global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
+{
+    extern __shared__ __align__(16) uint8_t lds_fast[];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / WAVE;
+    uint8_t *base = lds_fast + (uint64_t)wv * a.wave_bytes;
+    const uint32_t FS = 1u << a.flog2;
+    uint32_t *filt = reinterpret_cast<uint32_t *>(base);
+    uint64_t *beamA = reinterpret_cast<uint64_t *>(base + 4u * FS);
+    uint64_t *beamB = beamA + a.Lmax;
+    uint64_t *cand = beamB + a.Lmax;
+    uint64_t *cs = cand + 64 * IPL;
+    uint32_t *sel = reinterpret_cast<uint32_t *>(cs + 64 * IPL);
+    const uint32_t R = a.R;
+
+    while (true) {
+        uint32_t qi = 0;
+        if (lane == 0)
+            qi = atomicAdd(a.counter, 1u);
+        qi = __shfl(qi, 0);
+        if (qi >= a.nq)
+            break;
+        uint32_t L = a.Ls ? a.Ls[qi] : a.L;
+        L = max(1u, min(L, a.Lmax));
+        uint32_t k = a.ks ? a.ks[qi] : a.k;
+        k = min(k, min(L, a.out_k));
+        uint4 qv[NCH];
+        load_query<NCH>(a.queries + (uint64_t)qi * a.qstride, a.qbytes, qv);
+        const uint32_t nqn = norm_regs<DT, NCH>(qv);
+        for (uint32_t i = lane; i < FS; i += WAVE)
+            filt[i] = NBPG_SENT;
+        wave_sync();
+        uint64_t *beam = beamA, *beam2 = beamB;
+        {
+            const uint32_t e = a.entry;
+            const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)e * a.stride);
+            const float d = dist_from<DT>(a.norms[e], nqn, dot_row<DT, NCH>(r, qv));
+            if (lane == 0) {
+                beam[0] = make_key(d, e);
+                filt[fast_slot(e, a.flog2)] = e;
+            }
+        }
+        uint32_t bs = 1, nexp = 0, cmps = 1;
+        wave_sync();
+        while (nexp < a.limit) {
+            // pick up to W unexpanded entries, best first
+            uint32_t got = 0;
+            for (uint32_t i0 = 0; i0 < bs && got < a.W; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t e = i < bs ? beam[i] : FAST_FLAG;
+                const bool un = i < bs && !(e & FAST_FLAG);
+                const uint64_t m = ballot(un);
+                const uint32_t p = prefix_count(m);
+                if (un && got + p < a.W) {
+                    beam[i] = e | FAST_FLAG;
+                    sel[got + p] = key_id(e);
+                }
+                got = min(a.W, got + (uint32_t)__popcll(m));
+            }
+            if (got == 0)
+                break;
+            nexp += got;
+            wave_sync();
+            const uint64_t worst = bs >= L ? unflag(beam[L - 1]) : ~0ull;
+            // gather neighbour ids of the selected nodes and filter
+            uint32_t ids[IPL];
+            bool fresh[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                const uint32_t it = lane + WAVE * t;
+                uint32_t av = NBPG_SENT;
+                if (it < got * R) {
+                    const uint32_t node = sel[it / R];
+                    av = a.nbr[(uint64_t)node * R + (it % R)];
+                }
+                ids[t] = av;
+            }
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                bool f = ids[t] != NBPG_SENT;
+                if (f) {
+                    const uint32_t s = fast_slot(ids[t], a.flog2);
+                    f = filt[s] != ids[t];
+                    if (f)
+                        filt[s] = ids[t];
+                }
+                fresh[t] = f;
+            }
+            // distances of the fresh neighbours, all loads in flight together
+            float dd[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                dd[t] = 0.f;
+                if (fresh[t]) {
+                    const uint4 *r =
+                        reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride);
+                    dd[t] = dist_from<DT>(a.norms[ids[t]], nqn, dot_row<DT, NCH>(r, qv));
+                }
+            }
+            uint32_t nc = 0;
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                const uint64_t key = make_key(dd[t], ids[t]);
+                const bool keep = fresh[t] && key < worst;
+                const uint64_t m = ballot(keep);
+                cmps += (uint32_t)__popcll(ballot(fresh[t]));
+                if (keep)
+                    cand[nc + prefix_count(m)] = key;
+                nc += (uint32_t)__popcll(m);
+            }
+            wave_sync();
+            if (nc == 0)
+                continue;
+            // sort, drop duplicates and ids already in the beam
+            wave_rank_sort(cand, nc, cs, lane);
+            wave_sync();
+            uint32_t nu = 0;
+            for (uint32_t i0 = 0; i0 < nc; i0 += WAVE) {
+                const uint32_t i = i0 + lane;
+                const uint64_t v = i < nc ? cs[i] : 0;
+                bool keep = i < nc && (i == 0 || cs[i - 1] != v);
+                if (keep) {
+                    const uint32_t p = count_less_unflag(beam, bs, v);
+                    keep = !(p < bs && unflag(beam[p]) == v);
+                }
+                const uint64_t m = ballot(keep);
+                if (keep)
+                    cand[nu + prefix_count(m)] = v;
+                nu += (uint32_t)__popcll(m);
+            }
+            wave_sync();
+            if (nu == 0)
+                continue;
+            // merge into the second beam buffer by rank
+            for (uint32_t i = lane; i < bs; i += WAVE) {
+                const uint64_t e = beam[i];
+                const uint32_t p = i + count_less(cand, nu, unflag(e));
+                if (p < L)
+                    beam2[p] = e;
+            }
+            for (uint32_t j = lane; j < nu; j += WAVE) {
+                const uint64_t v = cand[j];
+                const uint32_t p = j + count_less_unflag(beam, bs, v);
+                if (p < L)
+                    beam2[p] = v;
+            }
+            bs = min(L, bs + nu);
+            {
+                uint64_t *t = beam;
+                beam = beam2;
+                beam2 = t;
+            }
+            wave_sync();
+        }
+        for (uint32_t j = lane; j < a.out_k; j += WAVE) {
+            const uint64_t o = (uint64_t)qi * a.out_k + j;
+            if (j < k && j < bs) {
+                const uint64_t e = unflag(beam[j]);
+                a.out_ids[o] = key_id(e);
+                if (a.out_dists)
+                    a.out_dists[o] = key_dist(e);
+            } else {
+                a.out_ids[o] = NBPG_SENT;
+                if (a.out_dists)
+                    a.out_dists[o] = __int_as_float(0x7f800000);
+            }
+        }
+        if (a.out_stats && lane == 0) {
+            a.out_stats[2ull * qi] = nexp;
+            a.out_stats[2ull * qi + 1] = cmps;
+        }
+        wave_sync();
+    }
+}
+
+__global__ void zero_counter_kernel(uint32_t *c) { c[threadIdx.x] = 0; }
+
+// --------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------
+
+static uint32_t ref_hash_size(uint32_t L)
+{
+    // query_params.h:243-247: 2^max(10, ceil(log2(L*L)) - 2)
+    uint32_t sq = L * L;
+    int bits = 0;
+    while ((1ull << bits) < (uint64_t)sq)
+        ++bits; // ceil(log2(sq)) for sq >= 1
+    bits -= 2;
+    if (bits < 10)
+        bits = 10;
+    return 1u << bits;
+}
+
+static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
+                      const uint32_t *sp, uint32_t sp_num, const nbpg_query_params *qp,
+                      uint32_t *d_ids, float *d_dists, uint32_t *d_stats, hipStream_t stream)
+{
+    SearchArgs a;
+    memset(&a, 0, sizeof(a));
+    a.vecs = ix->vecs;
+    a.norms = ix->norms;
+    a.nbr = ix->nbr;
+    a.n = ix->n;
+    a.stride = ix->stride;
+    a.R = ix->R;
+    a.queries = (const uint8_t *)d_queries;
+    a.qstride = qstride;
+    a.nq = nq;
+    a.qbytes = ix->dims;
+    for (uint32_t s = 0; s < sp_num; ++s)
+        a.sp[s] = sp[s];
+    a.sp_num = sp_num;
+    a.k = qp->k;
+    a.L = qp->L;
+    a.dlim = qp->degree_limit;
+    a.cut = qp->cut;
+    a.limit = qp->limit;
+    uint64_t mv = 8ull * qp->L; // beamSearch.h:279, 290
+    a.maxv = (uint32_t)std::min<uint64_t>(mv, 1u << 30);
+    a.H = ref_hash_size(qp->L);
+    a.out_ids = d_ids;
+    a.out_dists = d_dists;
+    a.out_stats = d_stats;
+    const RefLayout lo = ref_layout(a.L, a.R, a.maxv, a.H);
+    a.state_bytes = lo.total;
+    const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
+    const bool use_lds = lo.total <= lds_budget;
+    uint64_t grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (use_lds ? 32 : 8));
+    if (!use_lds) {
+        NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), grid * lo.total));
+        a.gscratch = (uint8_t *)ix->ws;
+    }
+    int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
+        if (use_lds)
+            hipLaunchKernelGGL((search_ref_kernel<DT.value, NCH.value, true>), dim3((uint32_t)grid),
+                               dim3(64), (uint32_t)lo.total, stream, a);
+        else
+            hipLaunchKernelGGL((search_ref_kernel<DT.value, NCH.value, false>),
+                               dim3((uint32_t)grid), dim3(64), 0, stream, a);
+    });
+    NBPG_TRY(rc);
+    HIP_TRY(hipGetLastError());
+    return NBPG_OK;
+}
+
+static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
+                       uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L,
+                       const uint32_t *d_k, uint32_t Lmax, uint32_t out_k, uint64_t limit,
+                       const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
+                       uint32_t *d_stats, hipStream_t stream)
+{
+    nbpg_fast_config c = {4, 1, 10, 0};
+    if (cfg)
+        c = *cfg;
+    if (c.waves_per_group < 1 || c.waves_per_group > 16 || c.expand_width < 1 ||
+        c.expand_width > 4 || c.filter_log2 < 8 || c.filter_log2 > 14)
+        return set_error(NBPG_EINVAL, "fast config out of range");
+    const uint32_t items = c.expand_width * ix->R;
+    const uint32_t ipl = (items + 63) / 64;
+    if (ipl > 4)
+        return set_error(NBPG_EUNSUP, "expand_width * R = %u > 256", items);
+    SearchArgs a;
+    memset(&a, 0, sizeof(a));
+    a.vecs = ix->vecs;
+    a.norms = ix->norms;
+    a.nbr = ix->nbr;
+    a.n = ix->n;
+    a.stride = ix->stride;
+    a.R = ix->R;
+    a.queries = (const uint8_t *)d_queries;
+    a.qstride = qstride;
+    a.nq = nq;
+    a.qbytes = ix->dims;
+    a.L = L;
+    a.k = k;
+    a.Ls = d_L;
+    a.ks = d_k;
+    a.Lmax = Lmax;
+    a.out_k = out_k;
+    a.W = c.expand_width;
+    a.flog2 = c.filter_log2;
+    a.entry = entry;
+    a.limit = limit;
+    a.wave_bytes = fast_wave_bytes(Lmax, ipl, c.filter_log2);
+    a.counter = ix->counter;
+    a.out_ids = d_ids;
+    a.out_dists = d_dists;
+    a.out_stats = d_stats;
+    const uint64_t lds = (uint64_t)a.wave_bytes * c.waves_per_group;
+    if (lds > (uint64_t)g_dev.lds_max)
+        return set_error(NBPG_EUNSUP, "fast search needs %llu B of LDS per group",
+                         (unsigned long long)lds);
+    uint32_t groups = c.groups;
+    if (groups == 0) {
+        // fill the LDS of every CU, at most 32 waves per CU
+        uint64_t per_cu = std::max<uint64_t>(1, (160ull * 1024) / lds);
+        per_cu = std::min<uint64_t>(per_cu, std::max<uint32_t>(1, 32 / c.waves_per_group));
+        groups = (uint32_t)(per_cu * g_dev.cu);
+    }
+    uint64_t need = (nq + c.waves_per_group - 1) / c.waves_per_group;
+    if (groups > need)
+        groups = (uint32_t)std::max<uint64_t>(1, need);
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, ix->counter);
+    int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
+        dim3 g(groups), b(64 * c.waves_per_group);
+        switch (ipl) {
+        case 1: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 1>), g, b, (uint32_t)lds, stream, a); break;
+        case 2: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 2>), g, b, (uint32_t)lds, stream, a); break;
+        case 3: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 3>), g, b, (uint32_t)lds, stream, a); break;
+        default: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 4>), g, b, (uint32_t)lds, stream, a); break;
+        }
+    });
+    NBPG_TRY(rc);
+    HIP_TRY(hipGetLastError());
+    return NBPG_OK;
+}
+
+} // namespace nbpg
+
+using namespace nbpg;
+
+extern "C" {
+
+int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,
+                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
+                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                uint32_t *d_out_stats, void *stream)
+{
+    if (!ix || !qp || (nq && (!d_queries || (qp->k && !d_out_ids))))
+        return set_error(NBPG_EINVAL, "null argument");
+    if (qp->L == 0)
+        return set_error(NBPG_EINVAL, "beam size L = 0");
+    if (qp->k > qp->L) // beamSearch.h:268-273
+        return set_error(NBPG_EINVAL, "k = %u > beam size L = %u", qp->k, qp->L);
+    if (query_stride < ix->dims)
+        return set_error(NBPG_EINVAL, "query stride %llu < row bytes %u",
+                         (unsigned long long)query_stride, ix->dims);
+    if (sp_num == 0 || !h_start_points || sp_num > 32 || sp_num > qp->L) // query_params.h:206-210
+        return set_error(NBPG_EINVAL, "start points: %u (1..min(32, L))", sp_num);
+    for (uint32_t s = 0; s < sp_num; ++s)
+        if (h_start_points[s] >= ix->n)
+            return set_error(NBPG_EINVAL, "start point %u >= n", h_start_points[s]);
+    if (nq == 0)
+        return NBPG_OK;
+    if (mode == NBPG_SEARCH_REF)
+        return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, d_out_ids,
+                          d_out_dists, d_out_stats, as_stream(stream));
+    if (mode == NBPG_SEARCH_FAST) {
+        if (sp_num != 1)
+            return set_error(NBPG_EUNSUP, "fast search takes one entry point");
+        if (nq >= 0xFFFFFFFFull)
+            return set_error(NBPG_EUNSUP, "nq too large");
+        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], qp->L, qp->k, nullptr,
+                           nullptr, qp->L, qp->k, qp->limit, fast, d_out_ids, d_out_dists,
+                           d_out_stats, as_stream(stream));
+    }
+    return set_error(NBPG_EINVAL, "search mode %d", mode);
+}
+
+int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                       uint64_t query_stride, const uint32_t *d_L, const uint32_t *d_k,
+                       uint32_t max_L, uint32_t out_k, uint32_t entry, const nbpg_fast_config *fast,
+                       uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats, void *stream)
+{
+    if (!ix || (nq && (!d_queries || !d_out_ids || !d_L || !d_k)))
+        return set_error(NBPG_EINVAL, "null argument");
+    if (max_L == 0 || out_k == 0 || out_k > max_L)
+        return set_error(NBPG_EINVAL, "max_L %u / out_k %u", max_L, out_k);
+    if (entry >= ix->n)
+        return set_error(NBPG_EINVAL, "entry %u >= n", entry);
+    if (query_stride < ix->dims)
+        return set_error(NBPG_EINVAL, "query stride too small");
+    if (nq == 0)
+        return NBPG_OK;
+    if (nq >= 0xFFFFFFFFull)
+        return set_error(NBPG_EUNSUP, "nq too large");
+    return search_fast(ix, d_queries, nq, query_stride, entry, 0, 0, d_L, d_k, max_L, out_k,
+                       ix->n, fast, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
+}
+
+} // extern "C"

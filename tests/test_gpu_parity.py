@@ -1,0 +1,220 @@
+"""HIP path vs the oracle / the reference's recorded outputs (needs a GPU)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from nbpg_amd import datagen as D
+
+pytestmark = pytest.mark.gpu
+
+
+def make_index(N, base, dtype, R, rows=None):
+    ix = N.Index(dtype, base.shape[0], base.shape[1], R)
+    ix.set_points(base)
+    if rows is not None:
+        ix.set_graph(rows)
+    return ix
+
+
+def dev_rows(N, a, stride):
+    """rows padded to `stride` bytes on the device"""
+    out = np.zeros((a.shape[0], stride), np.uint8)
+    out[:, :a.shape[1]] = a.view(np.uint8)
+    return N.to_device(out)
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    """5000 x 128 u8, 200 queries, R=32 graph built by the oracle."""
+    base, q = D.workload_small(5000, 200)
+    rows, _ = O.build_graph(base, O.U8, 32, 75, alpha=1.2, passes=1, num_split=0)
+    return base, q, rows
+
+
+def test_l2_batch_exact(gpu):
+    N = gpu
+    rng = np.random.default_rng(1)
+    for dtype, dims in [(N.U8, 128), (N.U8, 96), (N.I8, 96), (N.I8, 64), (N.U4X2, 32),
+                        (N.I4X2, 32), (N.U8, 200), (N.U8, 20)]:
+        base = rng.integers(0, 256, (1000, dims), dtype=np.uint8)
+        base[0] = 0
+        base[1] = 255
+        base[2] = 0x0F
+        q = rng.integers(0, 256, dims, dtype=np.uint8)
+        ix = make_index(N, base, dtype, 8)
+        ids = np.arange(1000, dtype=np.uint32)
+        out = N.l2_batch(ix, N.to_device(q), N.to_device(ids)).cpu().numpy()
+        odt = dtype if dtype != N.I4X2 else O.I4X2
+        ref = np.array([O.distance(odt, base[i], q) for i in ids], np.float32)
+        assert (out == ref).all(), (dtype, dims)
+        ix.close()
+
+
+def test_search_ref_matches_reference_golden(gpu, golden):
+    N = gpu
+    base, q, g1 = golden["base"], golden["queries"], golden["graph1"]
+    ix = make_index(N, base, N.U8, 24, g1)
+    qd = dev_rows(N, q, ix.row_stride)
+    for i, L in enumerate(golden["search_L"]):
+        ids, d, st = N.search(ix, qd, int(L), 10, mode=N.SEARCH_REF)
+        assert (ids.cpu().numpy().view(np.uint32) == golden["search_ids"][i]).all(), int(L)
+        assert (st.cpu().numpy().view(np.uint32) == golden["search_stats"][i]).all(), int(L)
+
+
+@pytest.mark.parametrize("L,k,dlim,limit", [(10, 10, 32, None), (16, 10, 32, None),
+                                             (32, 10, 32, None), (64, 10, 32, None),
+                                             (75, 1, 32, None), (128, 100, 32, None),
+                                             (256, 10, 32, None), (64, 10, 20, None),
+                                             (64, 10, 32, 100), (40, 0, 32, None)])
+def test_search_ref_matches_oracle(gpu, small, L, k, dlim, limit):
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    kk = max(k, 1)
+    ids, d, st = N.search(ix, qd, L, k, mode=N.SEARCH_REF, degree_limit=dlim, limit=limit)
+    oi, od, ost = O.search(rows, base, q, O.U8, L, k, degree_limit=dlim, limit=limit)
+    if k:
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all()
+        assert (d.cpu().numpy() == od).all()
+    assert (st.cpu().numpy().view(np.uint32) == ost).all()
+    ix.close()
+
+
+def test_search_ref_multi_start_and_types(gpu):
+    N = gpu
+    for dtype, dims, gen in [(N.I8, 64, "i8"), (N.U4X2, 32, "u4"), (N.U8, 96, "u8")]:
+        if gen == "i8":
+            base = D.mixture_i8(3000, dims, 30, 30.0, seed=5).view(np.uint8)
+            q = D.mixture_i8(64, dims, 30, 30.0, seed=6).view(np.uint8)
+        else:
+            base = D.mixture_u8(3000, dims, 30, 25.0, seed=5)
+            q = D.mixture_u8(64, dims, 30, 25.0, seed=6)
+        rows, _ = O.build_graph(base, dtype, 24, 48, alpha=1.2)
+        ix = make_index(N, base, dtype, 24, rows)
+        qd = dev_rows(N, q, ix.row_stride)
+        sp = [0, 17, 1234]
+        ids, d, st = N.search(ix, qd, 48, 10, mode=N.SEARCH_REF, start_points=sp)
+        G = O.Graph(rows.ctypes.data, rows.shape[0], 24)
+        oi = np.zeros((len(q), 10), np.uint32)
+        od = np.zeros((len(q), 10), np.float32)
+        ost = np.zeros((len(q), 2), np.uint32)
+        B, Q = O.points(base, dtype), O.points(q, dtype)
+        qp = O.QParams(10, 1.35, 48, base.shape[0], 24, 8 * 48)
+        spa = np.array(sp, np.uint32)
+        assert O.lib().nbpgo_search(O.C.byref(G), O.C.byref(B), O.C.byref(Q), spa.ctypes.data, 3,
+                                    O.C.byref(qp), oi.ctypes.data, od.ctypes.data,
+                                    ost.ctypes.data, 0) == 0
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), gen
+        assert (st.cpu().numpy().view(np.uint32) == ost).all(), gen
+        ix.close()
+
+
+def test_search_ref_rejects_bad_args(gpu, small):
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 8, 10)  # k > L
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 10, entry=base.shape[0])
+    ix.close()
+
+
+def test_search_fast_recall(gpu, small):
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    gt, _ = O.compgt(base, q, 100, O.U8, tie_mode=1)
+    for cfg in [(4, 1, 10, 0), (2, 2, 11, 0), (8, 4, 9, 0), (1, 1, 12, 3)]:
+        ids, d, st = N.search(ix, qd, 64, 10, mode=N.SEARCH_FAST, fast=cfg)
+        ids = ids.cpu().numpy().view(np.uint32)
+        d = d.cpu().numpy()
+        assert (ids < base.shape[0]).all()
+        assert (np.diff(d, axis=1) >= 0).all()
+        for j in range(len(q)):
+            assert len(set(ids[j].tolist())) == 10
+            ref = np.array([O.distance(O.U8, base[i], q[j]) for i in ids[j]], np.float32)
+            assert (ref == d[j]).all()
+        r = O.recall(ids, gt, base, q, O.U8, 10)
+        assert r >= 0.95, (cfg, r)
+    ix.close()
+
+
+def test_prune_matches_reference_golden(gpu, golden):
+    N = gpu
+    base = golden["base"]
+    ix = make_index(N, base, N.U8, 24)
+    P = len(golden["prune_nodes"])
+    cand = N.to_device(N.pack_cands(golden["prune_ids"], golden["prune_dists"]))
+    offs = N.to_device(golden["prune_offsets"].astype(np.int64))
+    nodes = N.to_device(golden["prune_nodes"])
+    alphas = N.to_device(golden["prune_alpha"].astype(np.float32))
+    ids, d, cnt = N.robust_prune_batch(ix, nodes, cand, offs, 16, alphas_dev=alphas)
+    ids = ids.cpu().numpy().view(np.uint32)
+    d = d.cpu().numpy()
+    cnt = cnt.cpu().numpy()
+    for t in range(P):
+        m = int(golden["prune_cnt"][t])
+        assert cnt[t] == m
+        assert (ids[t, :m] == golden["prune_out"][t, :m]).all()
+        assert (d[t, :m] == golden["prune_outd"][t, :m]).all()
+    ix.close()
+
+
+@pytest.mark.parametrize("alpha,cosa,R", [(1.2, 0.0, 64), (1.0, 0.0, 32), (0.0, 0.0, 16),
+                                          (1.2, 0.5, 64)])
+def test_prune_matches_oracle(gpu, small, alpha, cosa, R):
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32)
+    rng = np.random.default_rng(7)
+    P = 300
+    nodes = rng.integers(0, base.shape[0], P).astype(np.uint32)
+    offs = np.zeros(P + 1, np.int64)
+    allids, alld = [], []
+    for t in range(P):
+        n = int(rng.integers(1, 300))
+        ids = rng.integers(0, base.shape[0], n).astype(np.uint32)
+        if t % 4 == 0:
+            ids[n // 2] = nodes[t]
+        allids.append(ids)
+        alld.append(np.array([O.distance(O.U8, base[i], base[nodes[t]]) for i in ids], np.float32))
+        offs[t + 1] = offs[t] + n
+    cand = N.to_device(N.pack_cands(np.concatenate(allids), np.concatenate(alld)))
+    gi, gd, gc = N.robust_prune_batch(ix, N.to_device(nodes), cand, N.to_device(offs), R,
+                                      alpha=alpha, cos_angle=cosa)
+    gi, gd, gc = gi.cpu().numpy().view(np.uint32), gd.cpu().numpy(), gc.cpu().numpy()
+    for t in range(P):
+        oi, od = O.prune(base, O.U8, int(nodes[t]), allids[t], alld[t], R, alpha, cosa)
+        assert gc[t] == len(oi), t
+        assert (gi[t, :len(oi)] == oi).all(), t
+        assert (gd[t, :len(oi)] == od).all(), t
+    ix.close()
+
+
+@pytest.mark.parametrize("mode", [0])
+def test_compgt_exact(gpu, mode):
+    N = gpu
+    base, q = D.workload_small(20000, 100)
+    base[10000:10100] = base[:100]  # exact duplicates -> ties broken by id
+    q[:5] = base[:5]
+    for k in [1, 10, 100]:
+        ids, d = N.compgt(N.to_device(base), N.to_device(q), k, N.U8, mode)
+        oi, od = O.compgt(base, q, k, O.U8, tie_mode=1)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), k
+        assert (d.cpu().numpy() == od).all(), k
+
+
+def test_compgt_i8_and_4bit(gpu):
+    N = gpu
+    rng = np.random.default_rng(11)
+    for dtype, dims in [(N.I8, 64), (N.U4X2, 32), (N.U8, 96)]:
+        base = rng.integers(0, 256, (7000, dims), dtype=np.uint8)
+        q = rng.integers(0, 256, (33, dims), dtype=np.uint8)
+        ids, d = N.compgt(N.to_device(base), N.to_device(q), 50, dtype, N.GT_PLAIN)
+        oi, od = O.compgt(base, q, 50, dtype, tie_mode=1)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), dtype
+        assert (d.cpu().numpy() == od).all(), dtype
