@@ -1,0 +1,275 @@
+#!/usr/bin/env python3
+"""Headline benchmark: batched beam search on 1M x 128 u8 at recall@10 >= 0.95.
+
+One step = one persistent-kernel launch searching all query rows (10,000 by
+default) with inputs already resident in HBM.  Setup (untimed): seeded
+synthetic data, the NBPG graph built on the device (R=64, 2 passes, 4
+splits, dynamic prune, min_alpha 1.05), exact ground truth on the device,
+and a beam-width sweep that picks the smallest L reaching recall@10 >= 0.95.
+Beside the fast path the plain reference-exact path (one query per
+workgroup) is timed at its own smallest L with the same recall, and the
+reference's own qsearchAll (compiled from its sources, oracle/_ref) is timed
+on the host cores over a bounded sample as the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QPS @ recall@10>=0.95, 1M x128 u8: batched/served vs plain GPU; p99 ms"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench %7.1fs]" % (time.time() - T0), *a, file=sys.stderr, flush=True)
+
+
+T0 = time.time()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--L", type=int, default=0, help="fixed beam width (0 = sweep)")
+    ap.add_argument("--fast", default="auto",
+                    help="waves_per_group,expand_width,filter_log2,groups or 'auto' (autotuner)")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a hipGraph")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the search shards "
+              "by queries with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nbpg_amd as N
+    from nbpg_amd import autotune as AT
+    from nbpg_amd import datagen as D
+
+    N.init(0)
+    dev = N.device_info()
+    log("device", dev)
+    base, q = D.workload_1m(args.n, args.nq)
+    log("data", base.shape, q.shape, "zeros %.3f" % float((base == 0).mean()))
+    R = 64
+    ix = N.Index(N.U8, base.shape[0], base.shape[1], R)
+    ix.set_points(base)
+    base_d = N.to_device(base)
+    stride = ix.row_stride
+    qpad = np.zeros((q.shape[0], stride), np.uint8)
+    qpad[:, :q.shape[1]] = q
+    q_d = N.to_device(qpad)
+
+    t = time.time()
+    iso = N.build(ix, R, 70, VL=140, alpha=1.1, min_alpha=1.05, passes=2, num_split=4,
+                  dynamic_prune=True, sp_seed=1)
+    torch.cuda.synchronize()
+    build_s = time.time() - t
+    rows = ix.get_graph()
+    avg_deg = float(rows[:, 0].mean())
+    log("device NBPG build %.1fs, isolated %d, avg degree %.2f" % (build_s, iso, avg_deg))
+
+    t = time.time()
+    gt_ids, gt_d = N.compgt(base_d, torch.from_numpy(q).cuda(), 100, N.U8, N.GT_PLAIN)
+    torch.cuda.synchronize()
+    gt_s = time.time() - t
+    gt = gt_ids.cpu().numpy().view(np.uint32)
+    gtd = gt_d.cpu().numpy()
+    log("device ground truth top-100 in %.2fs" % gt_s)
+
+    def recall_at(ids, g):
+        return AT.recall_with_ties(ids, g, gtd[:len(g)], 10)
+
+    tune_table = None
+    if args.fast == "auto":
+        t = time.time()
+        best, tune_table = AT.tune(ix, q_d, gt, gtd, k=10, target=args.target, log=log)
+        fast = best["config"]
+        log("autotuner picked %s at L=%d in %.1fs" % (fast, best["L"], time.time() - t))
+        if not args.L:
+            args.L = best["L"]
+    else:
+        fast = tuple(int(x) for x in args.fast.split(","))
+    sweep = [args.L] if args.L else [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160,
+                                        200, 256]
+
+    def run_fast(L, out=None):
+        return N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, out=out)
+
+    def run_plain(L, out=None):
+        return N.search(ix, q_d, L, 10, mode=N.SEARCH_REF, cut=1.35, out=out)
+
+    def pick(runner, name):
+        for L in sweep:
+            ids, _, st = runner(L)
+            torch.cuda.synchronize()
+            r = recall_at(ids.cpu().numpy().view(np.uint32), gt)
+            log("%s L=%d recall@10 %.4f" % (name, L, r))
+            if r >= args.target:
+                return L, r, st.cpu().numpy()
+        return sweep[-1], r, st.cpu().numpy()
+
+    L_fast, rec_fast, st_fast = pick(run_fast, "fast")
+    sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
+    L_plain, rec_plain, st_plain = pick(run_plain, "plain")
+
+    def timed(runner, L, steps, warmup, graph):
+        out = runner(L)
+        step = lambda: runner(L, out)  # noqa: E731
+        if graph:
+            # one step captured in a hipGraph, replayed per step
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                runner(L, out)
+            torch.cuda.current_stream().wait_stream(s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                runner(L, out)
+            step = g.replay
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            step()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = np.array([a.elapsed_time(b) for a, b in ev])
+        return wall, per
+
+    wall, per = timed(run_fast, L_fast, args.steps, args.warmup, not args.no_graph)
+    nq = q.shape[0]
+    qps = nq * args.steps / wall
+    log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
+        (L_fast, qps, np.percentile(per, 50), np.percentile(per, 99)))
+    pwall, pper = timed(run_plain, L_plain, args.plain_steps, 1, False)
+    plain_qps = nq * args.plain_steps / pwall
+    log("plain L=%d: %.0f QPS" % (L_plain, plain_qps))
+
+    # roofline of the fast kernel: algorithmic bytes per launch
+    vis = st_fast[:, 0].astype(np.float64)
+    cmps = st_fast[:, 1].astype(np.float64)
+    dbytes = base.shape[1]
+    alg_bytes = float((vis * 4 * (1 + avg_deg)).sum() + (cmps * dbytes).sum() + nq * dbytes)
+    kernel_ms = float(per.mean())
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        global GTD
+        GTD = gtd
+        cpu = cpu_baseline(base, q, rows, gt, L_plain, args.cpu_seconds)
+
+    res = {
+        "metric": METRIC,
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded 1000-cluster Gaussian mixture, heavy-tailed dims)",
+        "config": {"workload": "1M x 128 u8 base, 10k queries, NBPG graph R=64 built on device, "
+                               "k=10, fast persistent search", "n": base.shape[0], "nq": nq,
+                   "L": L_fast, "fast_config": list(fast), "recall@10": round(rec_fast, 4),
+                   "hipgraph": not args.no_graph,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "p50_step_ms": round(float(np.percentile(per, 50)), 4),
+        "p99_step_ms": round(float(np.percentile(per, 99)), 4),
+        "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
+                  "speedup_fast_over_plain": round(qps / plain_qps, 2)},
+        "build_s": round(build_s, 2),
+        "groundtruth_s": round(gt_s, 3),
+        "mean_visited": round(float(vis.mean()), 1),
+        "mean_dist_cmps": round(float(cmps.mean()), 1),
+        "autotune": [{"config": list(r["config"]), "L": r["L"], "qps": round(r["qps"], 1)}
+                     for r in (tune_table or [])],
+    }
+    print(json.dumps(res), flush=True)
+
+
+GTD = None
+
+
+def cpu_baseline(base, q, rows, gt, L, seconds):
+    """The reference's qsearchAll on the host cores, bounded sample."""
+    try:
+        import oracle as O
+    except Exception as e:  # pragma: no cover
+        log("cpu baseline unavailable:", e)
+        return None
+    R = O.ref()
+    kind = "reference"
+    if R is None:
+        kind = "port"
+    k = 10
+    if kind == "reference":
+        h = R.nbpgr_open(base.ctypes.data, base.shape[0], base.shape[1], rows.ctypes.data,
+                         rows.shape[1] - 1)
+        threads = R.nbpgr_threads()
+
+        def run(nq):
+            qq = np.ascontiguousarray(q[:nq])
+            ids = np.zeros((nq, k), np.uint32)
+            st = np.zeros((nq, 2), np.uint32)
+            s = R.nbpgr_timed_search(h, qq.ctypes.data, nq, L, k, 1.35, base.shape[0],
+                                     rows.shape[1] - 1, 0, ids.ctypes.data, st.ctypes.data)
+            return s, ids
+    else:
+        threads = os.cpu_count()
+
+        def run(nq):
+            t = time.perf_counter()
+            ids, _, _ = O.search(rows, base, q[:nq], O.U8, L, k)
+            return time.perf_counter() - t, ids
+    # whole query batches, repeated until about `seconds` of host time
+    nq = len(q)
+    total, reps, ids = 0.0, 0, None
+    while total < seconds or reps == 0:
+        s, ids = run(nq)
+        total += s
+        reps += 1
+    if kind == "reference":
+        R.nbpgr_close(h)
+    from nbpg_amd.autotune import recall_with_ties
+    rec = recall_with_ties(ids, gt[:nq], GTD[:nq], 10)
+    log("cpu %s: %d x %d queries in %.2fs on %d threads, recall %.4f" %
+        (kind, reps, nq, total, threads, rec))
+    return {"value": round(reps * nq / total, 1), "unit": "queries/s", "cores": int(threads),
+            "kind": kind, "recall@10": round(rec, 4),
+            "sample": "%d passes over the %d queries (%.1f s), qsearchAll at L=%d, k=10, cut 1.35, "
+                      "same graph" % (reps, nq, total, L)}
+
+
+if __name__ == "__main__":
+    main()

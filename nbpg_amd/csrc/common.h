@@ -45,9 +45,24 @@ template <> struct dot_traits<NBPG_U4X2> {
 };
 template <> struct dot_traits<NBPG_I4X2> : dot_traits<NBPG_U4X2> {};
 
-// <x, q> over NCH 16-byte chunks; q held in registers
+// the query row as wave-uniform words: every lane holds the same query, so
+// it lives in scalar registers and feeds the dot instructions' scalar operand
+template <int NCH>
+__device__ __forceinline__ void to_uniform(const uint4 (&qv)[NCH], uint32_t (&qs)[4 * NCH])
+{
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        qs[4 * c + 0] = __builtin_amdgcn_readfirstlane(qv[c].x);
+        qs[4 * c + 1] = __builtin_amdgcn_readfirstlane(qv[c].y);
+        qs[4 * c + 2] = __builtin_amdgcn_readfirstlane(qv[c].z);
+        qs[4 * c + 3] = __builtin_amdgcn_readfirstlane(qv[c].w);
+    }
+}
+
+// <x, q> over NCH 16-byte chunks of the row; q wave-uniform
 template <int DT, int NCH>
-__device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row, const uint4 (&q)[NCH])
+__device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row,
+                                            const uint32_t (&q)[4 * NCH])
 {
     uint4 x[NCH];
 #pragma unroll
@@ -56,10 +71,10 @@ __device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row, const
     uint32_t acc = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        acc = dot_traits<DT>::dot(x[c].x, q[c].x, acc);
-        acc = dot_traits<DT>::dot(x[c].y, q[c].y, acc);
-        acc = dot_traits<DT>::dot(x[c].z, q[c].z, acc);
-        acc = dot_traits<DT>::dot(x[c].w, q[c].w, acc);
+        acc = dot_traits<DT>::dot(x[c].x, q[4 * c + 0], acc);
+        acc = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], acc);
+        acc = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], acc);
+        acc = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], acc);
     }
     return acc;
 }
@@ -75,6 +90,7 @@ __device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t do
 template <int DT, int NCH>
 __device__ __forceinline__ uint32_t norm_regs(const uint4 (&q)[NCH])
 {
+    // |q|^2 of a row held in vector registers
     uint32_t acc = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {

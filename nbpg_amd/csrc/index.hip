@@ -148,7 +148,9 @@ __global__ void l2_batch_kernel(const uint8_t *__restrict__ vecs, uint32_t strid
         }
         qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    const uint32_t nq = norm_regs<DT, NCH>(qv);
+    const uint32_t nq = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
+    uint32_t qs[4 * NCH];
+    to_uniform<NCH>(qv, qs);
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -158,7 +160,7 @@ __global__ void l2_batch_kernel(const uint8_t *__restrict__ vecs, uint32_t strid
         return;
     }
     const uint4 *r = reinterpret_cast<const uint4 *>(vecs + (uint64_t)id * stride);
-    out[i] = dist_from<DT>(norms[id], nq, dot_row<DT, NCH>(r, qv));
+    out[i] = dist_from<DT>(norms[id], nq, dot_row<DT, NCH>(r, qs));
 }
 
 // arbitrary packed rows: one thread per pair, plain byte loop

@@ -10,11 +10,9 @@
 // every later live candidate on 64 lanes at once, pruning when
 // alpha * d(p*, p') <= d(p, p') in double (prune.h:227-230) or, when
 // cos_angle > 1e-5, when the angle test fires (prune.h:232-238).
-#include <float.h>
-
 #include <algorithm>
 
-#include "internal.h"
+#include "prune_core.h"
 
 namespace nbpg {
 
@@ -39,35 +37,6 @@ struct PruneArgs {
     uint32_t *out_comps; // optional distance counts per node
     uint32_t *flag;      // bit 0: a node had > PRUNE_MAXC candidates; bit 1: bad id
 };
-
-// ascending bitonic sort of a[0:n2) (n2 a power of two) by one wavefront
-__device__ __forceinline__ void wave_bitonic(uint64_t *a, uint32_t n2, uint32_t lane)
-{
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = lane; i < n2; i += WAVE) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const uint64_t x = a[i], y = a[p];
-                    const bool up = (i & k) == 0;
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[p] = x;
-                    }
-                }
-            }
-            wave_sync();
-        }
-    }
-}
-
-__device__ __forceinline__ float cosine_at_p(float d_ps, float d_pp, float d_sp)
-{
-    const float b = sqrtf(d_pp), c = sqrtf(d_ps);
-    if (b == 0.f || c == 0.f)
-        return 1.f;
-    return (d_pp + d_ps - d_sp) / (2.f * b * c);
-}
 
 template <int DT, int NCH>
 __global__ __launch_bounds__(64 * PRUNE_WAVES) void prune_kernel(PruneArgs a)
@@ -112,78 +81,11 @@ __global__ __launch_bounds__(64 * PRUNE_WAVES) void prune_kernel(PruneArgs a)
         return;
     }
     wave_sync();
-    wave_bitonic(keys, n2, lane);
-    // collapse equal ids (equal ids carry equal keys once sorted)
-    uint32_t m = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
-        const uint32_t i = i0 + lane;
-        const uint64_t v = i < n ? keys[i] : 0;
-        const bool keep = i < n && (i == 0 || key_id(keys[i - 1]) != key_id(v));
-        const uint64_t km = ballot(keep);
-        wave_sync();
-        if (keep)
-            keys[m + prefix_count(km)] = v;
-        m += (uint32_t)__popcll(km);
-        wave_sync();
-    }
-    // drop trailing centre entries
-    while (m > 0 && key_id(keys[m - 1]) == p)
-        --m;
+    const uint32_t m = wave_sort_candidates(keys, n, p, lane);
     const double alpha = a.alphas ? (double)a.alphas[t] : a.alpha;
-    const double cosa = a.cos_angle;
-    const double eps = 1e-5;
-    uint32_t sel = 0, comps = 0;
-    if (alpha <= eps && cosa <= eps) { // prune.h:164-169
-        sel = min(m, a.R);
-        for (uint32_t i = lane; i < sel; i += WAVE) {
-            oid[i] = key_id(keys[i]);
-            if (od)
-                od[i] = key_dist(keys[i]);
-        }
-    } else {
-        uint32_t scan = 0;
-        while (sel < a.R && scan < m) {
-            const uint64_t ks = keys[scan];
-            const uint32_t ps = key_id(ks);
-            ++scan;
-            if (ps == p || ps == NBPG_SENT)
-                continue;
-            const float d_ps = key_dist(ks);
-            if (lane == 0) {
-                oid[sel] = ps;
-                if (od)
-                    od[sel] = d_ps;
-            }
-            ++sel;
-            uint4 sv[NCH];
-            const uint4 *sr = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ps * a.stride);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                sv[c] = sr[c];
-            const uint32_t ns = a.norms[ps];
-            for (uint32_t i0 = scan; i0 < m; i0 += WAVE) {
-                const uint32_t i = i0 + lane;
-                const uint64_t kv = i < m ? keys[i] : ~0ull;
-                const uint32_t pp = key_id(kv);
-                const bool live = i < m && pp != NBPG_SENT;
-                bool prune = false;
-                if (live) {
-                    const uint4 *r =
-                        reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)pp * a.stride);
-                    const float d_sp = dist_from<DT>(a.norms[pp], ns, dot_row<DT, NCH>(r, sv));
-                    const float d_pp = key_dist(kv);
-                    if (alpha > eps && alpha * (double)d_sp <= (double)d_pp)
-                        prune = true;
-                    else if (cosa > eps && (double)cosine_at_p(d_ps, d_pp, d_sp) > cosa)
-                        prune = true;
-                }
-                comps += (uint32_t)__popcll(ballot(live));
-                if (prune)
-                    keys[i] = (kv & 0xFFFFFFFF00000000ull) | NBPG_SENT;
-            }
-            wave_sync();
-        }
-    }
+    uint32_t comps = 0;
+    const uint32_t sel = wave_prune<DT, NCH>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride,
+                                             a.norms, oid, od, comps, lane);
     if (lane == 0) {
         a.out_counts[t] = sel;
         if (a.out_comps)

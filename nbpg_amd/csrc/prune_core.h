@@ -1,0 +1,145 @@
+// prune_core.h -- one wavefront's Prune::sort_candidates (prune.h:49-74)
+// and Prune::limitedSimilarityPrune (prune.h:158-209), shared by the
+// batched prune kernel and the device build's insert kernel.
+#pragma once
+#include <float.h>
+
+#include "internal.h"
+
+namespace nbpg {
+
+// ascending bitonic sort of a[0:n2) (n2 a power of two) by one wavefront
+__device__ __forceinline__ void wave_bitonic(uint64_t *a, uint32_t n2, uint32_t lane)
+{
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < n2; i += WAVE) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const uint64_t x = a[i], y = a[p];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[p] = x;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n)
+{
+    uint32_t n2 = 1;
+    while (n2 < n)
+        n2 <<= 1;
+    return n2;
+}
+
+// keys[0:n) hold (dist, id) keys with the centre p already keyed at FLT_MAX
+// (biased_less with dir_bias 0); keys must have room for pow2_at_least(n).
+// Sorts, collapses equal ids (dedupe_by_index: equal ids carry equal keys),
+// drops trailing centre entries.  Returns the new length.
+__device__ __forceinline__ uint32_t wave_sort_candidates(uint64_t *keys, uint32_t n, uint32_t p,
+                                                         uint32_t lane)
+{
+    const uint32_t n2 = pow2_at_least(n);
+    for (uint32_t i = n + lane; i < n2; i += WAVE)
+        keys[i] = ~0ull;
+    wave_sync();
+    wave_bitonic(keys, n2, lane);
+    uint32_t m = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
+        const uint32_t i = i0 + lane;
+        const uint64_t v = i < n ? keys[i] : 0;
+        const bool keep = i < n && (i == 0 || key_id(keys[i - 1]) != key_id(v));
+        const uint64_t km = ballot(keep);
+        wave_sync();
+        if (keep)
+            keys[m + prefix_count(km)] = v;
+        m += (uint32_t)__popcll(km);
+        wave_sync();
+    }
+    while (m > 0 && key_id(keys[m - 1]) == p)
+        --m;
+    return m;
+}
+
+__device__ __forceinline__ float cosine_at_p(float d_ps, float d_pp, float d_sp)
+{
+    // prune.h:218-225
+    const float b = sqrtf(d_pp), c = sqrtf(d_ps);
+    if (b == 0.f || c == 0.f)
+        return 1.f;
+    return (d_pp + d_ps - d_sp) / (2.f * b * c);
+}
+
+// limitedSimilarityPrune over sorted keys[0:m) for centre p; writes up to R
+// survivors to out_ids/out_d (lane 0 stores); returns the survivor count.
+// `comps` accumulates the distance evaluations (the reference's return).
+template <int DT, int NCH>
+__device__ uint32_t wave_prune(uint64_t *keys, uint32_t m, uint32_t p, uint32_t R, double alpha,
+                               double cosa, const uint8_t *__restrict__ vecs, uint32_t stride,
+                               const uint32_t *__restrict__ norms, uint32_t *out_ids, float *out_d,
+                               uint32_t &comps, uint32_t lane)
+{
+    const double eps = 1e-5;
+    uint32_t sel = 0;
+    if (alpha <= eps && cosa <= eps) { // prune.h:164-169
+        sel = min(m, R);
+        for (uint32_t i = lane; i < sel; i += WAVE) {
+            out_ids[i] = key_id(keys[i]);
+            if (out_d)
+                out_d[i] = key_dist(keys[i]);
+        }
+        return sel;
+    }
+    uint32_t scan = 0;
+    while (sel < R && scan < m) {
+        const uint64_t ks = keys[scan];
+        const uint32_t ps = key_id(ks);
+        ++scan;
+        if (ps == p || ps == NBPG_SENT)
+            continue;
+        const float d_ps = key_dist(ks);
+        if (lane == 0) {
+            out_ids[sel] = ps;
+            if (out_d)
+                out_d[sel] = d_ps;
+        }
+        ++sel;
+        uint4 sv[NCH];
+        const uint4 *sr = reinterpret_cast<const uint4 *>(vecs + (uint64_t)ps * stride);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            sv[c] = sr[c];
+        const uint32_t ns = norms[ps];
+        uint32_t ss[4 * NCH];
+        to_uniform<NCH>(sv, ss);
+        for (uint32_t i0 = scan; i0 < m; i0 += WAVE) {
+            const uint32_t i = i0 + lane;
+            const uint64_t kv = i < m ? keys[i] : ~0ull;
+            const uint32_t pp = key_id(kv);
+            const bool live = i < m && pp != NBPG_SENT;
+            bool prune = false;
+            if (live) {
+                const uint4 *r = reinterpret_cast<const uint4 *>(vecs + (uint64_t)pp * stride);
+                const float d_sp = dist_from<DT>(norms[pp], ns, dot_row<DT, NCH>(r, ss));
+                const float d_pp = key_dist(kv);
+                // prune.h:227-238, alpha test in double
+                if (alpha > eps && alpha * (double)d_sp <= (double)d_pp)
+                    prune = true;
+                else if (cosa > eps && (double)cosine_at_p(d_ps, d_pp, d_sp) > cosa)
+                    prune = true;
+            }
+            comps += (uint32_t)__popcll(ballot(live));
+            if (prune)
+                keys[i] = (kv & 0xFFFFFFFF00000000ull) | NBPG_SENT;
+        }
+        wave_sync();
+    }
+    return sel;
+}
+
+} // namespace nbpg

@@ -14,35 +14,9 @@
 //   tail are dropped before the merge.
 #include <algorithm>
 
-#include "internal.h"
+#include "ref_search.h"
 
 namespace nbpg {
-
-template <int NCH>
-__device__ __forceinline__ void load_query(const uint8_t *__restrict__ q, uint32_t qbytes,
-                                           uint4 (&qv)[NCH])
-{
-    const bool aligned = (reinterpret_cast<uintptr_t>(q) & 15) == 0;
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        if (aligned && (uint32_t)(c * 16 + 16) <= qbytes) {
-            qv[c] = reinterpret_cast<const uint4 *>(q)[c];
-        } else {
-            uint32_t w[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                uint32_t v = 0;
-                for (uint32_t b = 0; b < 4; ++b) {
-                    uint32_t off = c * 16 + t * 4 + b;
-                    if (off < qbytes)
-                        v |= (uint32_t)q[off] << (8 * b);
-                }
-                w[t] = v;
-            }
-            qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
-        }
-    }
-}
 
 struct SearchArgs {
     const uint8_t *vecs;
@@ -72,258 +46,44 @@ struct SearchArgs {
 };
 
 // --------------------------------------------------------------------------
-// reference-exact mode
+// reference-exact mode: one 64-lane workgroup per query
 // --------------------------------------------------------------------------
-
-struct RefLayout {
-    uint64_t fa, fb, unv, cand, cs, vis, total;
-};
-
-__host__ __device__ inline RefLayout ref_layout(uint32_t L, uint32_t R, uint32_t maxv, uint32_t H)
-{
-    RefLayout o;
-    uint64_t p = ((uint64_t)H * 4 + 15) & ~15ull;
-    const uint64_t fsz = 2ull * L + R, csz = (uint64_t)L + R + 64;
-    o.fa = p;
-    p += 8 * fsz;
-    o.fb = p;
-    p += 8 * fsz;
-    o.unv = p;
-    p += 8ull * L;
-    o.cand = p;
-    p += 8 * csz;
-    o.cs = p;
-    p += 8 * csz;
-    o.vis = p;
-    p += 8ull * maxv;
-    o.total = (p + 15) & ~15ull;
-    return o;
-}
-
-// dst[rank(src[i])] = src[i]; stable for equal keys; one wave
-__device__ __forceinline__ void wave_rank_sort(const uint64_t *src, uint32_t n, uint64_t *dst,
-                                               uint32_t lane)
-{
-    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
-        uint32_t i = i0 + lane;
-        uint64_t v = i < n ? src[i] : 0;
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-            uint64_t u = src[j];
-            r += (u < v) || (u == v && j < i);
-        }
-        if (i < n)
-            dst[r] = v;
-    }
-}
 
 template <int DT, int NCH, bool LDS>
 __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) uint8_t lds_state[];
     uint8_t *st = LDS ? lds_state : a.gscratch + (uint64_t)blockIdx.x * a.state_bytes;
-    const uint32_t L = a.L, R = a.R, H = a.H, maxv = a.maxv;
-    const RefLayout lo = ref_layout(L, R, maxv, H);
-    uint32_t *hash = reinterpret_cast<uint32_t *>(st);
-    uint64_t *F = reinterpret_cast<uint64_t *>(st + lo.fa);
-    uint64_t *NF = reinterpret_cast<uint64_t *>(st + lo.fb);
-    uint64_t *unv = reinterpret_cast<uint64_t *>(st + lo.unv);
-    uint64_t *cand = reinterpret_cast<uint64_t *>(st + lo.cand);
-    uint64_t *cs = reinterpret_cast<uint64_t *>(st + lo.cs);
-    uint64_t *vis = reinterpret_cast<uint64_t *>(st + lo.vis);
+    RefState s = ref_state(st, a.L, a.R, a.maxv, a.H);
+    const GraphView g = {a.vecs, a.norms, a.nbr, a.stride, a.R};
     const uint32_t lane = threadIdx.x;
-    const uint32_t lim = min(R, a.dlim);
-
     for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
         uint4 qv[NCH];
         load_query<NCH>(a.queries + qi * a.qstride, a.qbytes, qv);
-        const uint32_t nqn = norm_regs<DT, NCH>(qv);
-        auto dist_of = [&](uint32_t id) -> float {
-            const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)id * a.stride);
-            return dist_from<DT>(a.norms[id], nqn, dot_row<DT, NCH>(r, qv));
-        };
-        uint64_t *Fp = F, *NFp = NF;
-        for (uint32_t i = lane; i < H; i += WAVE)
-            hash[i] = NBPG_SENT;
-        __syncthreads();
-        // init_starting_points (query_params.h:203-221): filter in order
-        if (lane == 0)
-            for (uint32_t s = 0; s < a.sp_num; ++s) {
-                uint32_t slot = (uint32_t)(hash64_2(a.sp[s]) & (H - 1));
-                hash[slot] = a.sp[s];
-            }
-        for (uint32_t s = lane; s < a.sp_num; s += WAVE)
-            cand[s] = make_key(dist_of(a.sp[s]), a.sp[s]);
-        __syncthreads();
-        wave_rank_sort(cand, a.sp_num, Fp, lane);
-        __syncthreads();
-        for (uint32_t s = lane; s < a.sp_num; s += WAVE)
-            unv[s] = Fp[s];
-        uint32_t fs = a.sp_num, remain = a.sp_num, offset = 0, cand_n = 0, vs = 0;
-        uint64_t nvis = 0, cmps = a.sp_num;
-        __syncthreads();
-
-        while (remain > offset && nvis < a.limit) {
-            const uint64_t cur = unv[offset];
-            const uint32_t cid = key_id(cur);
-            // 1) neighbour filter, then 2) distances (beamSearch.h:71-93)
-            for (uint32_t j0 = 0; j0 < lim; j0 += WAVE) {
-                const uint32_t j = j0 + lane;
-                const uint32_t av = j < lim ? a.nbr[(uint64_t)cid * R + j] : NBPG_SENT;
-                const bool valid = av != NBPG_SENT;
-                const uint64_t vm = ballot(valid);
-                const uint32_t slot = valid ? (uint32_t)(hash64_2(av) & (H - 1)) : 0u;
-                const uint32_t hold = valid ? hash[slot] : 0u;
-                // lanes sharing a slot act in lane order: a lane sees the id
-                // written by the nearest lower lane of its slot, if any
-                bool has_pred = false, is_last = true;
-                uint32_t pred_a = 0;
-                for (uint64_t m = vm; m; m &= m - 1) {
-                    const int jl = __builtin_ctzll(m);
-                    const uint32_t sj = __builtin_amdgcn_readlane(slot, jl);
-                    const uint32_t aj = __builtin_amdgcn_readlane(av, jl);
-                    if (valid && sj == slot) {
-                        if (jl < (int)lane) {
-                            has_pred = true;
-                            pred_a = aj;
-                        } else if (jl > (int)lane) {
-                            is_last = false;
-                        }
-                    }
-                }
-                const bool seen = has_pred ? (pred_a == av) : (hold == av);
-                __syncthreads();
-                if (valid && is_last)
-                    hash[slot] = av;
-                const bool fresh = valid && !seen;
-                const uint64_t um = ballot(fresh);
-                if (fresh)
-                    cand[cand_n + prefix_count(um)] = make_key(dist_of(av), av);
-                const uint32_t nf = (uint32_t)__popcll(um);
-                cand_n += nf;
-                cmps += nf;
-                __syncthreads();
-            }
-            // 3) sorted insertion of `cur` into visited (beamSearch.h:95-117)
-            uint32_t c = 0;
-            for (uint32_t i = lane; i < vs; i += WAVE)
-                c += vis[i] <= cur;
-            const uint32_t pos = wave_sum(c);
-            ++nvis;
-            const uint32_t nvs = vs < maxv ? vs + 1 : vs;
-            if (pos + 1 < nvs) {
-                for (int32_t top = (int32_t)nvs - 2; top >= (int32_t)pos; top -= WAVE) {
-                    const int32_t idx = top - (int32_t)lane;
-                    uint64_t v = 0;
-                    if (idx >= (int32_t)pos)
-                        v = vis[idx];
-                    __syncthreads();
-                    if (idx >= (int32_t)pos)
-                        vis[idx + 1] = v;
-                    __syncthreads();
-                }
-            }
-            if (pos < nvs && lane == 0)
-                vis[pos] = cur;
-            vs = nvs;
-            __syncthreads();
-            // 4) delayed merge (beamSearch.h:119-125)
-            if (cand_n == 0 ||
-                (a.limit >= 2ull * L && cand_n < L / 8 && offset + 1 < remain)) {
-                ++offset;
-                continue;
-            }
-            offset = 0;
-            // sort + unique by id (beamSearch.h:128-130)
-            wave_rank_sort(cand, cand_n, cs, lane);
-            __syncthreads();
-            uint32_t cu = 0;
-            for (uint32_t i0 = 0; i0 < cand_n; i0 += WAVE) {
-                const uint32_t i = i0 + lane;
-                const uint64_t v = i < cand_n ? cs[i] : 0;
-                const bool keep = i < cand_n && (i == 0 || key_id(cs[i - 1]) != key_id(v));
-                const uint64_t km = ballot(keep);
-                if (keep)
-                    cand[cu + prefix_count(km)] = v;
-                cu += (uint32_t)__popcll(km);
-            }
-            __syncthreads();
-            // set_union(frontier, candidates) truncated to L (beamSearch.h:132-138)
-            uint32_t nb = 0;
-            for (uint32_t i0 = 0; i0 < cu; i0 += WAVE) {
-                const uint32_t i = i0 + lane;
-                const uint64_t v = i < cu ? cand[i] : 0;
-                const bool keep = i < cu && !contains(Fp, fs, v);
-                const uint64_t km = ballot(keep);
-                if (keep)
-                    cs[nb + prefix_count(km)] = v;
-                nb += (uint32_t)__popcll(km);
-            }
-            __syncthreads();
-            for (uint32_t i = lane; i < fs; i += WAVE) {
-                const uint64_t v = Fp[i];
-                const uint32_t p = i + count_less(cs, nb, v);
-                if (p < L)
-                    NFp[p] = v;
-            }
-            for (uint32_t j = lane; j < nb; j += WAVE) {
-                const uint64_t v = cs[j];
-                const uint32_t p = j + count_less(Fp, fs, v);
-                if (p < L)
-                    NFp[p] = v;
-            }
-            uint32_t nfs = min(L, fs + nb);
-            __syncthreads();
-            // cut at cut * d(k-th) (beamSearch.h:140-149)
-            if (a.k > 0 && nfs > a.k) {
-                const float thr = (float)(a.cut * (double)key_dist(NFp[a.k]));
-                const uint64_t tk = make_key(thr, 0u);
-                uint32_t cc = 0;
-                for (uint32_t i = lane; i < nfs; i += WAVE)
-                    cc += NFp[i] <= tk;
-                const uint32_t ub = wave_sum(cc);
-                nfs = max(ub, fs);
-            }
-            {
-                uint64_t *t = Fp;
-                Fp = NFp;
-                NFp = t;
-            }
-            fs = nfs;
-            cand_n = 0;
-            // unvisited = frontier[0:min(fs,L)] \ visited (beamSearch.h:155-160)
-            const uint32_t fl = min(fs, L);
-            uint32_t r = 0;
-            for (uint32_t i0 = 0; i0 < fl; i0 += WAVE) {
-                const uint32_t i = i0 + lane;
-                const uint64_t v = i < fl ? Fp[i] : 0;
-                const bool keep = i < fl && !contains(vis, vs, v);
-                const uint64_t km = ballot(keep);
-                if (keep)
-                    unv[r + prefix_count(km)] = v;
-                r += (uint32_t)__popcll(km);
-            }
-            remain = r;
-            __syncthreads();
-        }
+        const uint32_t nqn = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
+        uint32_t qs[4 * NCH];
+        to_uniform<NCH>(qv, qs);
+        const RefOut o = ref_search<DT, NCH>(
+            s, g, qs, nqn, [&](uint32_t t) { return a.sp[t]; }, a.sp_num, a.k, a.cut, a.limit,
+            a.dlim, NBPG_SENT, lane);
         // frontier[0:k] (beamSearch.h:249-254)
         for (uint32_t j = lane; j < a.k; j += WAVE) {
-            const uint64_t o = qi * a.k + j;
-            if (j < fs) {
-                a.out_ids[o] = key_id(Fp[j]);
+            const uint64_t oi = qi * a.k + j;
+            if (j < o.fs) {
+                a.out_ids[oi] = key_id(o.F[j]);
                 if (a.out_dists)
-                    a.out_dists[o] = key_dist(Fp[j]);
+                    a.out_dists[oi] = key_dist(o.F[j]);
             } else {
-                a.out_ids[o] = NBPG_SENT;
+                a.out_ids[oi] = NBPG_SENT;
                 if (a.out_dists)
-                    a.out_dists[o] = __int_as_float(0x7f800000);
+                    a.out_dists[oi] = __int_as_float(0x7f800000);
             }
         }
         if (a.out_stats && lane == 0) {
-            a.out_stats[2 * qi] = (uint32_t)nvis;
-            a.out_stats[2 * qi + 1] = (uint32_t)cmps;
+            a.out_stats[2 * qi] = (uint32_t)o.nvis;
+            a.out_stats[2 * qi + 1] = (uint32_t)o.cmps;
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -389,7 +149,9 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
         k = min(k, min(L, a.out_k));
         uint4 qv[NCH];
         load_query<NCH>(a.queries + (uint64_t)qi * a.qstride, a.qbytes, qv);
-        const uint32_t nqn = norm_regs<DT, NCH>(qv);
+        const uint32_t nqn = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
+        uint32_t qs[4 * NCH];
+        to_uniform<NCH>(qv, qs);
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
         wave_sync();
@@ -397,7 +159,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
         {
             const uint32_t e = a.entry;
             const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)e * a.stride);
-            const float d = dist_from<DT>(a.norms[e], nqn, dot_row<DT, NCH>(r, qv));
+            const float d = dist_from<DT>(a.norms[e], nqn, dot_row<DT, NCH>(r, qs));
             if (lane == 0) {
                 beam[0] = make_key(d, e);
                 filt[fast_slot(e, a.flog2)] = e;
@@ -457,7 +219,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
                 if (fresh[t]) {
                     const uint4 *r =
                         reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride);
-                    dd[t] = dist_from<DT>(a.norms[ids[t]], nqn, dot_row<DT, NCH>(r, qv));
+                    dd[t] = dist_from<DT>(a.norms[ids[t]], nqn, dot_row<DT, NCH>(r, qs));
                 }
             }
             uint32_t nc = 0;
@@ -541,19 +303,6 @@ __global__ void zero_counter_kernel(uint32_t *c) { c[threadIdx.x] = 0; }
 // --------------------------------------------------------------------------
 // host side
 // --------------------------------------------------------------------------
-
-static uint32_t ref_hash_size(uint32_t L)
-{
-    // query_params.h:243-247: 2^max(10, ceil(log2(L*L)) - 2)
-    uint32_t sq = L * L;
-    int bits = 0;
-    while ((1ull << bits) < (uint64_t)sq)
-        ++bits; // ceil(log2(sq)) for sq >= 1
-    bits -= 2;
-    if (bits < 10)
-        bits = 10;
-    return 1u << bits;
-}
 
 static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
                       const uint32_t *sp, uint32_t sp_num, const nbpg_query_params *qp,

@@ -16,12 +16,13 @@ def cluster_centers_u8(n_clusters, dims, seed, lo=20.0, hi=200.0):
     return _rng(seed).uniform(lo, hi, size=(n_clusters, dims)).astype(np.float32)
 
 
-def mixture_u8(n, dims, n_clusters, sigma, seed, centers_seed=None, scale=None,
+def mixture_u8(n, dims, n_clusters, sigma, seed, centers_seed=None, scale=None, shift=0.0,
                chunk=1 << 18):
-    """u8 rows: round(center + sigma * N(0,1) * scale) clipped to 0..255.
+    """u8 rows: round(scale * (center + sigma * N(0,1)) - shift) clipped to 0..255.
 
-    `scale` is an optional per-dimension multiplier (heavy-tailed workloads);
-    centres come from `centers_seed` so base and queries share them."""
+    `scale` is an optional per-dimension multiplier (heavy-tailed workloads)
+    and `shift` moves mass below zero so clipping yields exact zeros; centres
+    come from `centers_seed` so base and queries share them."""
     centers = cluster_centers_u8(n_clusters, dims, centers_seed if centers_seed is not None else 7)
     g = _rng(seed)
     out = np.empty((n, dims), np.uint8)
@@ -31,15 +32,15 @@ def mixture_u8(n, dims, n_clusters, sigma, seed, centers_seed=None, scale=None,
         noise = g.standard_normal(size=(e - s, dims), dtype=np.float32) * np.float32(sigma)
         x = centers[lab] + noise
         if scale is not None:
-            x = (x - 110.0) * scale + 110.0
+            x = x * scale - np.float32(shift)
         np.clip(np.rint(x), 0, 255, out=x)
         out[s:e] = x.astype(np.uint8)
     return out
 
 
 def heavy_tail_scale(dims, seed=11, sigma=0.5):
-    """Per-dimension lognormal scale used by the 1M x 128 workload."""
-    return _rng(seed).lognormal(0.0, sigma, size=dims).astype(np.float32) * np.float32(1.6)
+    """Per-dimension lognormal(0, sigma) scale used by the 1M x 128 workload."""
+    return _rng(seed).lognormal(0.0, sigma, size=dims).astype(np.float32)
 
 
 def mixture_i8(n, dims, n_clusters, sigma, seed, centers_seed=5, chunk=1 << 18):
@@ -64,8 +65,9 @@ def workload_small(n=100_000, nq=1_000, dims=128):
 
 
 def workload_1m(n=1_000_000, nq=10_000, dims=128):
-    """1M x 128 u8, 1000 clusters, heavy-tailed per-dimension scale."""
+    """1M x 128 u8, 1000 clusters (sigma 25), lognormal(0, 0.5) per-dimension
+    scale and a shift that leaves about 30% of the entries at zero."""
     sc = heavy_tail_scale(dims)
-    base = mixture_u8(n, dims, 1000, 25.0, seed=3, centers_seed=9, scale=sc)
-    queries = mixture_u8(nq, dims, 1000, 25.0, seed=4, centers_seed=9, scale=sc)
+    base = mixture_u8(n, dims, 1000, 25.0, seed=3, centers_seed=9, scale=sc, shift=66.0)
+    queries = mixture_u8(nq, dims, 1000, 25.0, seed=4, centers_seed=9, scale=sc, shift=66.0)
     return base, queries

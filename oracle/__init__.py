@@ -133,6 +133,16 @@ def ref():
         R.nbpgr_compress.restype = None
         R.nbpgr_compress.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint32,
                                      C.c_void_p]
+        R.nbpgr_open.restype = C.c_void_p
+        R.nbpgr_open.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]
+        R.nbpgr_timed_search.restype = C.c_double
+        R.nbpgr_timed_search.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, C.c_double, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, C.c_void_p, C.c_void_p]
+        R.nbpgr_close.restype = None
+        R.nbpgr_close.argtypes = [C.c_void_p]
+        R.nbpgr_threads.restype = C.c_int
+        R.nbpgr_threads.argtypes = []
         _ref = R
     return _ref
 

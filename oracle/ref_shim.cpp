@@ -13,6 +13,8 @@
 #include <string>
 #include <cstring>
 #include <random>
+#include <chrono>
+#include <omp.h>
 #include <vector>
 
 #include "utils/Point/point.h"
@@ -179,5 +181,50 @@ void nbpgr_compress(const float *y, uint64_t n, uint32_t d, int is_signed, uint3
     for (uint64_t i = 0; i < n; ++i)
         std::memcpy(out + i * od, cp[i].data(), od);
 }
+
+// persistent u8 index for timing: the copy into the reference's containers
+// happens once in nbpgr_open; nbpgr_timed_search brackets qsearchAll alone,
+// as checkRecall does (check_nn_recall.h:74-76)
+struct RefHandle {
+    PointRange<Point<uint8_t>> B;
+    Graph<uint32_t> G;
+    uint32_t dims;
+};
+
+void *nbpgr_open(const uint8_t *base, uint64_t n, uint32_t dims, const uint32_t *rows, uint32_t R)
+{
+    auto *h = new RefHandle{make_points<uint8_t>(base, n, dims), make_graph(rows, n, R), dims};
+    return h;
+}
+
+double nbpgr_timed_search(void *hv, const uint8_t *queries, uint64_t nq, uint32_t L, uint32_t k,
+                          double cut, uint64_t limit, uint32_t degree_limit, uint32_t entry,
+                          uint32_t *out_ids, uint32_t *out_stats)
+{
+    auto *h = reinterpret_cast<RefHandle *>(hv);
+    auto Q = make_points<uint8_t>(queries, nq, h->dims);
+    QueryStatic stats(nq);
+    stats.clear();
+    uint32_t sp = entry;
+    std::function<std::pair<uint32_t *, uint32_t>(uint32_t)> get_sp =
+        [&](uint32_t) { return std::make_pair(&sp, (uint32_t)1); };
+    QueryParams QP(k, L, cut, limit, degree_limit);
+    auto t0 = std::chrono::steady_clock::now();
+    auto res = qsearchAll<uint32_t>(Q, Q, h->G, h->B, h->B, stats, get_sp, QP);
+    auto t1 = std::chrono::steady_clock::now();
+    for (uint64_t i = 0; i < nq; ++i) {
+        for (uint32_t j = 0; j < k; ++j)
+            out_ids[i * k + j] = j < res[i].size() ? res[i][j] : 0xFFFFFFFFu;
+        if (out_stats) {
+            out_stats[2 * i] = stats.visited[i];
+            out_stats[2 * i + 1] = stats.distances[i];
+        }
+    }
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+void nbpgr_close(void *hv) { delete reinterpret_cast<RefHandle *>(hv); }
+
+int nbpgr_threads(void) { return omp_get_max_threads(); }
 
 } // extern "C"

@@ -218,3 +218,96 @@ def test_compgt_i8_and_4bit(gpu):
         oi, od = O.compgt(base, q, 50, dtype, tie_mode=1)
         assert (ids.cpu().numpy().view(np.uint32) == oi).all(), dtype
         assert (d.cpu().numpy() == od).all(), dtype
+
+
+def live(rows):
+    out = rows.copy()
+    R = rows.shape[1] - 1
+    out[:, 1:][np.arange(R)[None, :] >= rows[:, :1]] = 0
+    return out
+
+
+def test_build_matches_reference_golden(gpu, golden):
+    """Device build == the reference's own Vamana binary, row for row."""
+    N = gpu
+    base = golden["base"]
+    ix = make_index(N, base, N.U8, 24)
+    N.build(ix, 24, 48, alpha=1.2, passes=1, num_split=0)
+    assert (live(ix.get_graph()) == live(golden["graph1"])).all()
+    ix.close()
+    ix = make_index(N, base, N.U8, 24)
+    N.build(ix, 24, 40, VL=60, alpha=1.1, min_alpha=1.05, passes=2, num_split=1,
+            dynamic_prune=True)
+    assert (live(ix.get_graph()) == live(golden["graph2"])).all()
+    ix.close()
+
+
+@pytest.mark.parametrize("dtype,dims,kw", [
+    ("u8", 128, dict(R=32, L=64, alpha=1.2, passes=1, num_split=0)),
+    ("u8", 128, dict(R=32, L=50, alpha=1.1, min_alpha=1.05, passes=2, num_split=4,
+                     dynamic_prune=True, sp_seed=7)),
+    ("i8", 96, dict(R=24, L=40, alpha=1.2, passes=1, num_split=0, rand_sp=True)),
+    ("u4", 32, dict(R=16, L=32, alpha=1.2, passes=2, num_split=0, top_degree=8)),
+])
+def test_build_matches_oracle(gpu, dtype, dims, kw):
+    N = gpu
+    if dtype == "i8":
+        base = D.mixture_i8(4000, dims, 40, 30.0, seed=31).view(np.uint8)
+        t = N.I8
+    else:
+        base = D.mixture_u8(4000, dims, 40, 25.0, seed=31)
+        t = N.U8 if dtype == "u8" else N.U4X2
+    R = kw.pop("R")
+    L = kw.pop("L")
+    ix = make_index(N, base, t, R)
+    iso = N.build(ix, R, L, **kw)
+    rows, oiso = O.build_graph(base, t, R, L, **kw)
+    assert iso == oiso
+    assert (live(ix.get_graph()) == live(rows)).all()
+    ix.close()
+
+
+def test_fast_search_hipgraph_replay(gpu, small):
+    """One fast search captured in a hipGraph; every replay (with eager work
+    in between) reproduces the eager result."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    e_ids, e_d, e_st = N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST)
+    e_ids, e_d = e_ids.cpu().numpy(), e_d.cpu().numpy()
+    out = N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST, out=out)
+    for rep in range(4):
+        out[0].fill_(-1)
+        out[1].fill_(-1.0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert (out[0].cpu().numpy() == e_ids).all(), rep
+        assert (out[1].cpu().numpy() == e_d).all(), rep
+        N.search(ix, qd, 32, 10, mode=N.SEARCH_REF)  # eager work between replays
+        torch.cuda.synchronize()
+    ix.close()
+
+
+def test_fast_and_reference_modes_agree_mostly(gpu, small):
+    """Both modes are best-first searches over the same graph: at a beam
+    width with high recall their top-10 sets nearly coincide."""
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    a, _, _ = N.search(ix, qd, 64, 10, mode=N.SEARCH_FAST)
+    b, _, _ = N.search(ix, qd, 64, 10, mode=N.SEARCH_REF)
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    agree = np.mean([len(set(x) & set(y)) / 10 for x, y in zip(a, b)])
+    assert agree > 0.97, agree
+    ix.close()
