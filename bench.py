@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -59,6 +61,17 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "3 copies of 1 GiB", "bytes_read_per_copy": 1 << 30,
+                          "bytes_written_per_copy": 1 << 30}), flush=True)
+        return
     import nbpg_amd as N
     from nbpg_amd import autotune as AT
     from nbpg_amd import datagen as D
