@@ -69,6 +69,8 @@ typedef struct {
     uint32_t expand_width;    /* beam entries expanded per step: 1..4 */
     uint32_t filter_log2;     /* LDS visited-filter slots = 1 << filter_log2 (8..14) */
     uint32_t groups;          /* persistent workgroups; 0 = derive from occupancy */
+    uint32_t row_split;       /* 0: one lane gathers a whole vector row; 1: one lane per
+                                 16-byte chunk, a load instruction covers whole rows */
 } nbpg_fast_config;
 
 /* build_vamana's flags (Vamana.cpp:14-85, nsw.h:547-551) */

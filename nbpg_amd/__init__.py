@@ -34,7 +34,7 @@ class QueryParams(C.Structure):
 
 class FastConfig(C.Structure):
     _fields_ = [("waves_per_group", C.c_uint32), ("expand_width", C.c_uint32),
-                ("filter_log2", C.c_uint32), ("groups", C.c_uint32)]
+                ("filter_log2", C.c_uint32), ("groups", C.c_uint32), ("row_split", C.c_uint32)]
 
 
 class BuildParams(C.Structure):

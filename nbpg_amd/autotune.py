@@ -15,26 +15,32 @@ import numpy as np
 import nbpg_amd as N
 
 DEFAULT_SPACE = {
-    "waves_per_group": (2, 4, 8),
-    "expand_width": (1, 2, 4),
-    "filter_log2": (9, 10, 11),
+    "waves_per_group": (2, 4),
+    "expand_width": (1, 2),
+    "filter_log2": (10, 11, 12),
+    "row_split": (0, 1),
+    "waves_per_cu": (16, 24, 0),  # 0: as many as LDS and registers allow
 }
 DEFAULT_L = (10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256)
 
 
 def recall_with_ties(ids, gt_ids, gt_dists, k=10):
     """recall@k counting ground-truth ties at the k-th distance as hits
-    (the reference's calRecall, check_nn_recall.h:126-167)."""
-    hit = 0
-    for i in range(len(ids)):
-        g = gt_ids[i]
-        n = k
-        last = gt_dists[i, k - 1]
-        while n < len(g) and gt_dists[i, n] == last:
-            n += 1
-        rs = set(ids[i, :k].tolist())
-        hit += sum(1 for x in g[:n].tolist() if x in rs)
-    return hit / (k * len(ids))
+    (the reference's calRecall, check_nn_recall.h:126-167): ground-truth
+    entries past k whose distance equals the k-th one are also admissible."""
+    ids = np.asarray(ids)[:, :k].astype(np.int64)
+    g = np.asarray(gt_ids).astype(np.int64)
+    gd = np.asarray(gt_dists)
+    last = gd[:, k - 1:k]
+    admissible = np.zeros(g.shape, bool)
+    admissible[:, :k] = True
+    if g.shape[1] > k:
+        # the ties must continue the run right after position k-1
+        run = np.cumprod(gd[:, k:] == last, axis=1).astype(bool)
+        admissible[:, k:] = run
+    # a ground-truth id counts when it appears among the k reported ids
+    hit = (g[:, :, None] == ids[:, None, :]).any(axis=2) & admissible
+    return float(hit.sum()) / (k * len(ids))
 
 
 def _time(fn, reps):
@@ -58,9 +64,11 @@ def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAU
     nq = q_dev.shape[0]
     table = []
     best = None
-    for wpg, w, fl in itertools.product(space["waves_per_group"], space["expand_width"],
-                                        space["filter_log2"]):
-        cfg = (wpg, w, fl, 0)
+    cu = N.device_info()["cu"]
+    for wpg, w, fl, split, wpc in itertools.product(space["waves_per_group"], space["expand_width"],
+                                                   space["filter_log2"], space["row_split"],
+                                                   space["waves_per_cu"]):
+        cfg = (wpg, w, fl, (cu * wpc) // wpg if wpc else 0, split)
         try:
             chosen = None
             for L in Ls:

@@ -87,6 +87,31 @@ __device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t do
     return (float)(int32_t)(s * dot_traits<DT>::scale);
 }
 
+// distance to a row without a stored norm: |x|^2 comes from the same loaded
+// chunks (one more dot per word), so a distance is exactly one row gather
+template <int DT, int NCH>
+__device__ __forceinline__ float dist_row(const uint4 *__restrict__ row, const uint32_t (&q)[4 * NCH],
+                                         uint32_t nq)
+{
+    uint4 x[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        x[c] = row[c];
+    uint32_t xq = 0, xx = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        xq = dot_traits<DT>::dot(x[c].x, q[4 * c + 0], xq);
+        xq = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], xq);
+        xq = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], xq);
+        xq = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], xq);
+        xx = dot_traits<DT>::dot(x[c].x, x[c].x, xx);
+        xx = dot_traits<DT>::dot(x[c].y, x[c].y, xx);
+        xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
+        xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
+    }
+    return dist_from<DT>(xx, nq, xq);
+}
+
 template <int DT, int NCH>
 __device__ __forceinline__ uint32_t norm_regs(const uint4 (&q)[NCH])
 {

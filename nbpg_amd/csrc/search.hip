@@ -15,35 +15,9 @@
 #include <algorithm>
 
 #include "ref_search.h"
+#include "search_args.h"
 
 namespace nbpg {
-
-struct SearchArgs {
-    const uint8_t *vecs;
-    const uint32_t *norms;
-    const uint32_t *nbr;
-    uint64_t n;
-    uint32_t stride, R;
-    const uint8_t *queries;
-    uint64_t qstride, nq;
-    uint32_t qbytes;
-    // reference mode
-    uint32_t sp[32];
-    uint32_t sp_num;
-    uint32_t k, L, dlim, maxv, H;
-    double cut;
-    uint64_t limit;
-    uint8_t *gscratch;
-    uint64_t state_bytes;
-    // fast mode
-    const uint32_t *Ls, *ks;
-    uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;
-    uint32_t *counter;
-    // outputs
-    uint32_t *out_ids;
-    float *out_dists;
-    uint32_t *out_stats;
-};
 
 // --------------------------------------------------------------------------
 // reference-exact mode: one 64-lane workgroup per query
@@ -91,10 +65,6 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
 // fast mode
 // --------------------------------------------------------------------------
 
-#define FAST_FLAG 0x80000000ull // bit 31 of the id word: entry expanded
-
-__device__ __forceinline__ uint64_t unflag(uint64_t k) { return k & ~FAST_FLAG; }
-
 __device__ __forceinline__ uint32_t count_less_unflag(const uint64_t *a, uint32_t n, uint64_t v)
 {
     uint32_t lo = 0, hi = n;
@@ -106,11 +76,6 @@ __device__ __forceinline__ uint32_t count_less_unflag(const uint64_t *a, uint32_
             hi = mid;
     }
     return lo;
-}
-
-__device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
-{
-    return (id * 0x9E3779B1u) >> (32 - flog2);
 }
 
 __host__ __device__ inline uint32_t fast_wave_bytes(uint32_t Lmax, uint32_t ipl, uint32_t flog2)
@@ -362,7 +327,7 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
                        const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
                        uint32_t *d_stats, hipStream_t stream)
 {
-    nbpg_fast_config c = {4, 1, 10, 0};
+    nbpg_fast_config c = {2, 1, 11, 0, 1};
     if (cfg)
         c = *cfg;
     if (c.waves_per_group < 1 || c.waves_per_group > 16 || c.expand_width < 1 ||
@@ -394,7 +359,16 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     a.flog2 = c.filter_log2;
     a.entry = entry;
     a.limit = limit;
-    a.wave_bytes = fast_wave_bytes(Lmax, ipl, c.filter_log2);
+    // beams of up to 256 entries live in registers (search_reg.hip); larger
+    // beams use the LDS-resident kernel below
+    const bool reg = Lmax <= 4 * WAVE;
+    const uint32_t E = Lmax <= WAVE ? 1 : (Lmax <= 2 * WAVE ? 2 : 4);
+    if (reg && ((qstride & 3) || (reinterpret_cast<uintptr_t>(d_queries) & 3)))
+        return set_error(NBPG_EINVAL, "query rows must be 4-byte aligned");
+    const bool split = c.row_split != 0;
+    // register beam: filter (+ the row-split kernel's compact list of 64*ipl ids)
+    a.wave_bytes = reg ? (4u << c.filter_log2) + (split ? 256u * ipl : 0u)
+                       : fast_wave_bytes(Lmax, ipl, c.filter_log2);
     a.counter = ix->counter;
     a.out_ids = d_ids;
     a.out_dists = d_dists;
@@ -406,7 +380,7 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     uint32_t groups = c.groups;
     if (groups == 0) {
         // fill the LDS of every CU, at most 32 waves per CU
-        uint64_t per_cu = std::max<uint64_t>(1, (160ull * 1024) / lds);
+        uint64_t per_cu = std::max<uint64_t>(1, (uint64_t)g_dev.lds_max / lds);
         per_cu = std::min<uint64_t>(per_cu, std::max<uint32_t>(1, 32 / c.waves_per_group));
         groups = (uint32_t)(per_cu * g_dev.cu);
     }
@@ -414,8 +388,10 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     if (groups > need)
         groups = (uint32_t)std::max<uint64_t>(1, need);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, ix->counter);
+    dim3 g(groups), b(64 * c.waves_per_group);
+    if (reg)
+        return launch_search_reg(ix->dtype, ix->nch, ipl, E, split, g, b, (uint32_t)lds, stream, a);
     int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
-        dim3 g(groups), b(64 * c.waves_per_group);
         switch (ipl) {
         case 1: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 1>), g, b, (uint32_t)lds, stream, a); break;
         case 2: hipLaunchKernelGGL((search_fast_kernel<DT.value, NCH.value, 2>), g, b, (uint32_t)lds, stream, a); break;

@@ -1,0 +1,46 @@
+// search_args.h -- launch arguments and helpers shared by the search kernels.
+#pragma once
+#include "internal.h"
+
+namespace nbpg {
+
+struct SearchArgs {
+    const uint8_t *vecs;
+    const uint32_t *norms;
+    const uint32_t *nbr;
+    uint64_t n;
+    uint32_t stride, R;
+    const uint8_t *queries;
+    uint64_t qstride, nq;
+    uint32_t qbytes;
+    // reference mode
+    uint32_t sp[32];
+    uint32_t sp_num;
+    uint32_t k, L, dlim, maxv, H;
+    double cut;
+    uint64_t limit;
+    uint8_t *gscratch;
+    uint64_t state_bytes;
+    // fast mode
+    const uint32_t *Ls, *ks;
+    uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;
+    uint32_t *counter;
+    // outputs
+    uint32_t *out_ids;
+    float *out_dists;
+    uint32_t *out_stats;
+};
+
+#define FAST_FLAG 0x80000000ull // bit 31 of the id word: entry expanded
+
+__device__ __forceinline__ uint64_t unflag(uint64_t k) { return k & ~FAST_FLAG; }
+
+__device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
+{
+    return (id * 0x9E3779B1u) >> (32 - flog2);
+}
+
+int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
+                      dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a);
+
+} // namespace nbpg

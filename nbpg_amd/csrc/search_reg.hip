@@ -1,0 +1,474 @@
+// search_reg.hip -- fast search with the beam held in vector registers.
+//
+// One wavefront per query, pulled from an atomic work counter (persistent
+// grid).  The beam of L <= 64*E entries lives in registers: lane i holds
+// entries i, 64+i, ... as (dist, id) keys in ascending order across lanes,
+// bit 31 of the id word marking an expanded entry.  Per step:
+//   select   the W best unexpanded entries (one ballot per register row)
+//   gather   their adjacency rows (one contiguous R*4-byte load each)
+//   filter   a lossy direct-mapped visited filter in LDS (as the reference's,
+//            query_params.h:223-230, without its lane-order semantics)
+//   score    exact distances of the unseen neighbours; |x|^2 is computed
+//            from the gathered row, so a distance is exactly one row gather
+//   insert   each neighbour that beats the beam tail: rank = ballot count
+//            of smaller keys, entries at or past the rank move up one lane
+// until no unexpanded entry is left in the beam.  No LDS is touched by the
+// beam, so a step has two dependent global round trips and no LDS chains.
+#include "search_args.h"
+
+namespace nbpg {
+
+__device__ __forceinline__ uint64_t shfl_up1(uint64_t v)
+{
+    const uint32_t lo = __shfl_up((uint32_t)v, 1), hi = __shfl_up((uint32_t)(v >> 32), 1);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int E>
+__device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t c, uint32_t L, uint32_t lane)
+{
+    bool dup = false;
+    uint32_t pos = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint64_t u = unflag(b[e]);
+        dup |= ballot(u == c) != 0;
+        pos += (uint32_t)__popcll(ballot(u < c));
+    }
+    if (dup || pos >= L)
+        return;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        uint64_t up = shfl_up1(b[e]);
+        const uint64_t last = readlane64(b[e], WAVE - 1);
+        if (lane == 0)
+            up = carry;
+        carry = last;
+        const uint32_t gp = lane + WAVE * e;
+        uint64_t nb = gp > pos ? up : (gp == pos ? c : b[e]);
+        if (gp >= L)
+            nb = ~0ull;
+        b[e] = nb;
+    }
+}
+
+template <int DT, int NCH, int IPL, int E>
+__global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
+{
+    extern __shared__ __align__(16) uint8_t lds_filter[];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / WAVE;
+    const uint32_t FS = 1u << a.flog2;
+    uint32_t *filt = reinterpret_cast<uint32_t *>(lds_filter) + (uint64_t)wv * FS;
+    const uint32_t R = a.R;
+    constexpr uint64_t EMPTY = ~0ull;
+
+    while (true) {
+        uint32_t qi = 0;
+        if (lane == 0)
+            qi = atomicAdd(a.counter, 1u);
+        qi = __builtin_amdgcn_readfirstlane(qi);
+        if (qi >= a.nq)
+            break;
+        uint32_t L = a.Ls ? a.Ls[qi] : a.L;
+        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, (uint32_t)(WAVE * E))));
+        uint32_t k = a.ks ? a.ks[qi] : a.k;
+        k = __builtin_amdgcn_readfirstlane(min(k, min(L, a.out_k)));
+        uint4 qv[NCH];
+        {
+            const uint8_t *qp = a.queries + (uint64_t)qi * a.qstride;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                uint32_t w[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    uint32_t v = 0;
+                    const uint32_t off = c * 16 + t * 4;
+                    if (off + 4 <= a.qbytes)
+                        v = *reinterpret_cast<const uint32_t *>(qp + off);
+                    else
+                        for (uint32_t bb = 0; bb < 4; ++bb)
+                            if (off + bb < a.qbytes)
+                                v |= (uint32_t)qp[off + bb] << (8 * bb);
+                    w[t] = v;
+                }
+                qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        }
+        const uint32_t nqn = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
+        uint32_t qs[4 * NCH];
+        to_uniform<NCH>(qv, qs);
+        for (uint32_t i = lane; i < FS; i += WAVE)
+            filt[i] = NBPG_SENT;
+        uint64_t b[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            b[e] = EMPTY;
+        {
+            const uint32_t en = a.entry;
+            const float d0 = dist_row<DT, NCH>(
+                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), qs, nqn);
+            wave_sync();
+            if (lane == 0) {
+                b[0] = make_key(d0, en);
+                filt[fast_slot(en, a.flog2)] = en;
+            }
+        }
+        uint32_t nexp = 0, cmps = 1;
+        wave_sync();
+        while (nexp < a.limit) {
+            // the W best unexpanded entries
+            uint32_t got = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t gp = lane + WAVE * e;
+                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
+                while (m && got < a.W) {
+                    const int l = __builtin_ctzll(m);
+                    const uint32_t id = __builtin_amdgcn_readlane((uint32_t)b[e], l);
+                    s0 = got == 0 ? id : s0;
+                    s1 = got == 1 ? id : s1;
+                    s2 = got == 2 ? id : s2;
+                    s3 = got == 3 ? id : s3;
+                    if ((int)lane == l)
+                        b[e] |= FAST_FLAG;
+                    ++got;
+                    m &= m - 1;
+                }
+            }
+            if (got == 0)
+                break;
+            nexp += got;
+            // gather adjacency rows, filter
+            uint32_t ids[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                const uint32_t it = lane + WAVE * t;
+                uint32_t av = NBPG_SENT;
+                if (it < got * R) {
+                    const uint32_t j = it / R;
+                    const uint32_t node = j == 0 ? s0 : (j == 1 ? s1 : (j == 2 ? s2 : s3));
+                    av = a.nbr[(uint64_t)node * R + (it - j * R)];
+                }
+                ids[t] = av;
+            }
+            bool fresh[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                bool f = ids[t] != NBPG_SENT;
+                if (f) {
+                    const uint32_t sl = fast_slot(ids[t], a.flog2);
+                    f = filt[sl] != ids[t];
+                    if (f)
+                        filt[sl] = ids[t];
+                }
+                fresh[t] = f;
+            }
+            // exact distances of the unseen neighbours, loads in flight together
+            uint64_t key[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                key[t] = EMPTY;
+                if (fresh[t])
+                    key[t] = make_key(
+                        dist_row<DT, NCH>(
+                            reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride),
+                            qs, nqn),
+                        ids[t]);
+                cmps += (uint32_t)__popcll(ballot(fresh[t]));
+            }
+            // beam tail
+            const uint32_t wl = L - 1;
+            uint64_t tail = EMPTY;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if ((uint32_t)e == wl / WAVE)
+                    tail = readlane64(b[e], (int)(wl % WAVE));
+            const uint64_t worst = unflag(tail);
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                uint64_t m = ballot(key[t] < worst);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    beam_insert<E>(b, readlane64(key[t], l), L, lane);
+                    m &= m - 1;
+                }
+            }
+            wave_sync();
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t gp = lane + WAVE * e;
+            if (gp < a.out_k) {
+                const uint64_t o = (uint64_t)qi * a.out_k + gp;
+                const bool ok = gp < k && b[e] != EMPTY;
+                const uint64_t v = unflag(b[e]);
+                a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
+                if (a.out_dists)
+                    a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);
+            }
+        }
+        if (a.out_stats && lane == 0) {
+            a.out_stats[2ull * qi] = nexp;
+            a.out_stats[2ull * qi + 1] = cmps;
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// row-split variant: the unseen neighbours are compacted into an LDS list and
+// scored LPR lanes per row, one 16-byte chunk per lane, so one load
+// instruction covers RPI whole rows (8 rows of 128 B) instead of one chunk
+// of 64 different rows; the per-lane partials |x_c|^2 - 2<x_c, q_c> are
+// summed over the row's lanes with ds_swizzle xor steps.
+// ---------------------------------------------------------------------------
+
+template <int M> __device__ __forceinline__ uint32_t swz_xor(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
+}
+
+template <int NCH> struct row_split_shape {
+    static constexpr int LPR = NCH <= 2 ? 2 : (NCH <= 4 ? 4 : (NCH <= 8 ? 8 : 16));
+    static constexpr int RPI = WAVE / LPR;
+};
+
+template <int LPR> __device__ __forceinline__ uint32_t row_sum(uint32_t p)
+{
+    p += swz_xor<1>(p);
+    p += swz_xor<2>(p);
+    if (LPR >= 8)
+        p += swz_xor<4>(p);
+    if (LPR >= 16)
+        p += swz_xor<8>(p);
+    return p;
+}
+
+template <int DT, int NCH, int IPL, int E>
+__global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
+{
+    constexpr int LPR = row_split_shape<NCH>::LPR, RPI = row_split_shape<NCH>::RPI;
+    constexpr int G = 8; // row groups in flight per lane
+    extern __shared__ __align__(16) uint8_t lds_split[];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / WAVE;
+    const uint32_t FS = 1u << a.flog2;
+    uint32_t *filt = reinterpret_cast<uint32_t *>(lds_split + (uint64_t)wv * a.wave_bytes);
+    uint32_t *cl = filt + FS;
+    const uint32_t R = a.R;
+    const uint32_t c = lane % LPR, rsub = lane / LPR;
+    constexpr uint64_t EMPTY = ~0ull;
+
+    while (true) {
+        uint32_t qi = 0;
+        if (lane == 0)
+            qi = atomicAdd(a.counter, 1u);
+        qi = __builtin_amdgcn_readfirstlane(qi);
+        if (qi >= a.nq)
+            break;
+        uint32_t L = a.Ls ? a.Ls[qi] : a.L;
+        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, (uint32_t)(WAVE * E))));
+        uint32_t k = a.ks ? a.ks[qi] : a.k;
+        k = __builtin_amdgcn_readfirstlane(min(k, min(L, a.out_k)));
+        // this lane's 16-byte chunk of the query (zero past the row)
+        uint4 qc = make_uint4(0, 0, 0, 0);
+        {
+            const uint8_t *qp = a.queries + (uint64_t)qi * a.qstride + 16 * c;
+            uint32_t w[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                uint32_t v = 0;
+                const uint32_t off = 16 * c + 4 * t;
+                if (c < NCH && off + 4 <= a.qbytes)
+                    v = *reinterpret_cast<const uint32_t *>(qp + 4 * t);
+                else if (c < NCH)
+                    for (uint32_t bb = 0; bb < 4; ++bb)
+                        if (off + bb < a.qbytes)
+                            v |= (uint32_t)qp[4 * t + bb] << (8 * bb);
+                w[t] = v;
+            }
+            qc = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        uint32_t nqp = 0;
+        nqp = dot_traits<DT>::dot(qc.x, qc.x, nqp);
+        nqp = dot_traits<DT>::dot(qc.y, qc.y, nqp);
+        nqp = dot_traits<DT>::dot(qc.z, qc.z, nqp);
+        nqp = dot_traits<DT>::dot(qc.w, qc.w, nqp);
+        const uint32_t nqn = __builtin_amdgcn_readfirstlane(row_sum<LPR>(nqp));
+        for (uint32_t i = lane; i < FS; i += WAVE)
+            filt[i] = NBPG_SENT;
+        uint64_t b[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            b[e] = EMPTY;
+
+        // scores cl[0:nf) and inserts the ones that beat the beam tail
+        auto score_list = [&](uint32_t nf) {
+            for (uint32_t base = 0; base < nf; base += RPI * G) {
+                uint4 x[G];
+                uint32_t id[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const uint32_t r = base + g * RPI + rsub;
+                    id[g] = r < nf ? cl[r] : 0u;
+                    x[g] = make_uint4(0, 0, 0, 0);
+                    if (r < nf && c < NCH)
+                        x[g] = *reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)id[g] * a.stride +
+                                                                16 * c);
+                }
+                const uint32_t wl = L - 1;
+                uint64_t tail = EMPTY;
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    if ((uint32_t)e == wl / WAVE)
+                        tail = readlane64(b[e], (int)(wl % WAVE));
+                const uint64_t worst = unflag(tail);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const uint32_t r = base + g * RPI + rsub;
+                    uint32_t xx = 0, xq = 0;
+                    xx = dot_traits<DT>::dot(x[g].x, x[g].x, xx);
+                    xx = dot_traits<DT>::dot(x[g].y, x[g].y, xx);
+                    xx = dot_traits<DT>::dot(x[g].z, x[g].z, xx);
+                    xx = dot_traits<DT>::dot(x[g].w, x[g].w, xx);
+                    xq = dot_traits<DT>::dot(x[g].x, qc.x, xq);
+                    xq = dot_traits<DT>::dot(x[g].y, qc.y, xq);
+                    xq = dot_traits<DT>::dot(x[g].z, qc.z, xq);
+                    xq = dot_traits<DT>::dot(x[g].w, qc.w, xq);
+                    const uint32_t part = row_sum<LPR>(xx - 2u * xq);
+                    const float d = (float)(int32_t)((nqn + part) * dot_traits<DT>::scale);
+                    const uint64_t key = make_key(d, id[g]);
+                    uint64_t m = ballot(c == 0 && r < nf && key < worst);
+                    while (m) {
+                        const int l = __builtin_ctzll(m);
+                        beam_insert<E>(b, readlane64(key, l), L, lane);
+                        m &= m - 1;
+                    }
+                }
+            }
+        };
+
+        wave_sync();
+        if (lane == 0) {
+            cl[0] = a.entry;
+            filt[fast_slot(a.entry, a.flog2)] = a.entry;
+        }
+        wave_sync();
+        score_list(1);
+        uint32_t nexp = 0, cmps = 1;
+        wave_sync();
+        while (nexp < a.limit) {
+            uint32_t got = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t gp = lane + WAVE * e;
+                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
+                while (m && got < a.W) {
+                    const int l = __builtin_ctzll(m);
+                    const uint32_t idn = __builtin_amdgcn_readlane((uint32_t)b[e], l);
+                    s0 = got == 0 ? idn : s0;
+                    s1 = got == 1 ? idn : s1;
+                    s2 = got == 2 ? idn : s2;
+                    s3 = got == 3 ? idn : s3;
+                    if ((int)lane == l)
+                        b[e] |= FAST_FLAG;
+                    ++got;
+                    m &= m - 1;
+                }
+            }
+            if (got == 0)
+                break;
+            nexp += got;
+            uint32_t ids[IPL];
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                const uint32_t it = lane + WAVE * t;
+                uint32_t av = NBPG_SENT;
+                if (it < got * R) {
+                    const uint32_t j = it / R;
+                    const uint32_t node = j == 0 ? s0 : (j == 1 ? s1 : (j == 2 ? s2 : s3));
+                    av = a.nbr[(uint64_t)node * R + (it - j * R)];
+                }
+                ids[t] = av;
+            }
+            uint32_t nf = 0;
+#pragma unroll
+            for (int t = 0; t < IPL; ++t) {
+                bool f = ids[t] != NBPG_SENT;
+                if (f) {
+                    const uint32_t sl = fast_slot(ids[t], a.flog2);
+                    f = filt[sl] != ids[t];
+                    if (f)
+                        filt[sl] = ids[t];
+                }
+                const uint64_t m = ballot(f);
+                if (f)
+                    cl[nf + prefix_count(m)] = ids[t];
+                nf += (uint32_t)__popcll(m);
+            }
+            cmps += nf;
+            wave_sync();
+            score_list(nf);
+            wave_sync();
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t gp = lane + WAVE * e;
+            if (gp < a.out_k) {
+                const uint64_t o = (uint64_t)qi * a.out_k + gp;
+                const bool ok = gp < k && b[e] != EMPTY;
+                const uint64_t v = unflag(b[e]);
+                a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
+                if (a.out_dists)
+                    a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);
+            }
+        }
+        if (a.out_stats && lane == 0) {
+            a.out_stats[2ull * qi] = nexp;
+            a.out_stats[2ull * qi + 1] = cmps;
+        }
+        wave_sync();
+    }
+}
+
+int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
+                      dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a)
+{
+    int rc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
+        constexpr int D = DT.value, C = NCH.value;
+#define NBPG_REG_LAUNCH(I, EE)                                                                     \
+    do {                                                                                           \
+        if (split)                                                                                 \
+            hipLaunchKernelGGL((search_split_kernel<D, C, I, EE>), grid, block, lds, stream, a);  \
+        else                                                                                       \
+            hipLaunchKernelGGL((search_reg_kernel<D, C, I, EE>), grid, block, lds, stream, a);    \
+    } while (0)
+        const uint32_t sel = (ipl <= 1 ? 0 : ipl <= 2 ? 1 : 2) * 3 + (E <= 1 ? 0 : E <= 2 ? 1 : 2);
+        switch (sel) {
+        case 0: NBPG_REG_LAUNCH(1, 1); break;
+        case 1: NBPG_REG_LAUNCH(1, 2); break;
+        case 2: NBPG_REG_LAUNCH(1, 4); break;
+        case 3: NBPG_REG_LAUNCH(2, 1); break;
+        case 4: NBPG_REG_LAUNCH(2, 2); break;
+        case 5: NBPG_REG_LAUNCH(2, 4); break;
+        case 6: NBPG_REG_LAUNCH(4, 1); break;
+        case 7: NBPG_REG_LAUNCH(4, 2); break;
+        default: NBPG_REG_LAUNCH(4, 4); break;
+        }
+#undef NBPG_REG_LAUNCH
+    });
+    NBPG_TRY(rc);
+    HIP_TRY(hipGetLastError());
+    return NBPG_OK;
+}
+
+} // namespace nbpg

@@ -128,7 +128,8 @@ def test_search_fast_recall(gpu, small):
     ix = make_index(N, base, N.U8, 32, rows)
     qd = dev_rows(N, q, ix.row_stride)
     gt, _ = O.compgt(base, q, 100, O.U8, tie_mode=1)
-    for cfg in [(4, 1, 10, 0), (2, 2, 11, 0), (8, 4, 9, 0), (1, 1, 12, 3)]:
+    for cfg in [(4, 1, 10, 0), (2, 2, 11, 0), (8, 4, 9, 0), (1, 1, 12, 3), (2, 1, 11, 0, 1),
+                (4, 2, 10, 0, 1), (2, 4, 10, 512, 1)]:
         ids, d, st = N.search(ix, qd, 64, 10, mode=N.SEARCH_FAST, fast=cfg)
         ids = ids.cpu().numpy().view(np.uint32)
         d = d.cpu().numpy()
@@ -310,4 +311,26 @@ def test_fast_and_reference_modes_agree_mostly(gpu, small):
     a, b = a.cpu().numpy(), b.cpu().numpy()
     agree = np.mean([len(set(x) & set(y)) / 10 for x, y in zip(a, b)])
     assert agree > 0.97, agree
+    ix.close()
+
+
+@pytest.mark.parametrize("L,cfg", [(300, (2, 1, 10, 0)), (200, (4, 2, 11, 0)), (100, (2, 4, 9, 0)),
+                                   (200, (2, 1, 11, 0, 1)), (100, (4, 4, 10, 0, 1))])
+def test_search_fast_large_beams(gpu, small, L, cfg):
+    """Register beams (L <= 256, 1-4 entries per lane) and the LDS-resident
+    beam (L > 256) return exact distances, sorted, and high recall."""
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    gt, _ = O.compgt(base, q, 100, O.U8, tie_mode=1)
+    ids, d, st = N.search(ix, qd, L, 50, mode=N.SEARCH_FAST, fast=cfg)
+    ids = ids.cpu().numpy().view(np.uint32)
+    d = d.cpu().numpy()
+    assert (np.diff(d, axis=1) >= 0).all()
+    for j in range(0, len(q), 7):
+        ref = np.array([O.distance(O.U8, base[i], q[j]) for i in ids[j]], np.float32)
+        assert (ref == d[j]).all()
+        assert len(set(ids[j].tolist())) == 50
+    assert O.recall(ids, gt, base, q, O.U8, 50) >= 0.97
     ix.close()
