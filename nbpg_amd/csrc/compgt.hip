@@ -27,7 +27,8 @@ struct GtArgs {
     uint64_t n, nq;
     uint32_t dims, k, slices;
     uint64_t slice_rows;
-    uint64_t *partial; // [nq][slices][k] keys
+    uint64_t row_stride; // bytes between base rows (dims, or a multiple to sample)
+    uint64_t *partial;   // [nq][slices][k] keys
 };
 
 // ascending bitonic sort of a[0:n2) by the whole workgroup
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(GT_THREADS) void compgt_plain_kernel(GtArgs a)
         const uint64_t r = rb + tid;
         if (r < r1) {
             uint32_t x[NW];
-            const uint32_t *rw = reinterpret_cast<const uint32_t *>(a.base + r * a.dims);
+            const uint32_t *rw = reinterpret_cast<const uint32_t *>(a.base + r * a.row_stride);
 #pragma unroll
             for (int w = 0; w < NW; ++w)
                 x[w] = (uint32_t)w < words ? rw[w] : 0u;
@@ -208,8 +209,53 @@ __global__ __launch_bounds__(256) void compgt_merge_kernel(const uint64_t *__res
 }
 
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                uint32_t dims, uint32_t k, uint64_t *partial, uint32_t slices, uint64_t slice_rows,
-                hipStream_t stream);
+                uint32_t dims, uint32_t k, uint32_t *out_ids, float *out_dists, hipStream_t stream);
+
+// plain scan + slice merge into ids / dists (dists may be NULL).  Base row r
+// is at base + r * row_stride and is reported as id r.
+int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                        const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
+                        uint32_t *ids, float *dists, hipStream_t s)
+{
+    const uint32_t nch = nch_for_bytes(dims);
+    const uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
+    // enough workgroups to fill the device, slices of at least 2048 rows
+    uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / std::max<uint64_t>(1, tiles));
+    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 2048));
+    slices = std::max<uint64_t>(1, std::min<uint64_t>(slices, 4096 / k));
+    const uint64_t slice_rows = (n + slices - 1) / slices;
+    uint64_t *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, nq * slices * k * 8, s) != hipSuccess)
+        return set_error(NBPG_ENOMEM, "compgt partial buffer");
+    GtArgs a;
+    a.base = base;
+    a.queries = queries;
+    a.n = n;
+    a.nq = nq;
+    a.dims = dims;
+    a.k = k;
+    a.slices = (uint32_t)slices;
+    a.slice_rows = slice_rows;
+    a.row_stride = row_stride;
+    a.partial = partial;
+    int rc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
+        hipLaunchKernelGGL((compgt_plain_kernel<DT.value, NCH.value>),
+                           dim3((uint32_t)tiles, (uint32_t)slices), dim3(GT_THREADS), 0, s, a);
+    });
+    if (rc == NBPG_OK && hipGetLastError() != hipSuccess)
+        rc = set_error(NBPG_EHIP, "compgt launch");
+    if (rc == NBPG_OK) {
+        uint32_t m = (uint32_t)(slices * k), n2 = 1;
+        while (n2 < m)
+            n2 <<= 1;
+        hipLaunchKernelGGL(compgt_merge_kernel, dim3((uint32_t)nq), dim3(256), n2 * 8, s, partial,
+                           (uint32_t)slices, k, ids, dists);
+        if (hipGetLastError() != hipSuccess)
+            rc = set_error(NBPG_EHIP, "compgt merge launch");
+    }
+    hipFreeAsync(partial, s);
+    return rc;
+}
 
 } // namespace nbpg
 
@@ -228,58 +274,19 @@ extern "C" int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void
     if (dims % 4 != 0 || (reinterpret_cast<uintptr_t>(d_base) & 3) ||
         (reinterpret_cast<uintptr_t>(d_queries) & 3))
         return set_error(NBPG_EUNSUP, "rows must be whole 4-byte words (dims %u)", dims);
-    const uint32_t nch = nch_for_bytes(dims);
-    if (nch == 0)
+    if (nch_for_bytes(dims) == 0)
         return set_error(NBPG_EUNSUP, "rows of %u bytes (<= 256 supported)", dims);
     if (n >= 0xFFFFFFFFull)
         return set_error(NBPG_EUNSUP, "n too large");
     if (nq == 0)
         return NBPG_OK;
     hipStream_t s = as_stream(stream);
-    const uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
-    // enough workgroups to fill the device, slices of at least 2048 rows
-    uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / std::max<uint64_t>(1, tiles));
-    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 2048));
-    slices = std::min<uint64_t>(slices, 4096 / k);
     if (mode == NBPG_GT_MFMA)
-        slices = std::max<uint64_t>(1, std::min<uint64_t>(slices, 4096 / k));
-    const uint64_t slice_rows = (n + slices - 1) / slices;
-    uint64_t *partial = nullptr;
-    if (hipMallocAsync((void **)&partial, nq * slices * k * 8, s) != hipSuccess)
-        return set_error(NBPG_ENOMEM, "compgt partial buffer");
-    int rc = NBPG_OK;
-    if (mode == NBPG_GT_MFMA) {
-        rc = compgt_mfma(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq, dims, k,
-                         partial, (uint32_t)slices, slice_rows, s);
-    } else if (mode == NBPG_GT_PLAIN) {
-        GtArgs a;
-        a.base = (const uint8_t *)d_base;
-        a.queries = (const uint8_t *)d_queries;
-        a.n = n;
-        a.nq = nq;
-        a.dims = dims;
-        a.k = k;
-        a.slices = (uint32_t)slices;
-        a.slice_rows = slice_rows;
-        a.partial = partial;
-        rc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
-            hipLaunchKernelGGL((compgt_plain_kernel<DT.value, NCH.value>),
-                               dim3((uint32_t)tiles, (uint32_t)slices), dim3(GT_THREADS), 0, s, a);
-        });
-        if (rc == NBPG_OK && hipGetLastError() != hipSuccess)
-            rc = set_error(NBPG_EHIP, "compgt launch");
-    } else {
-        rc = set_error(NBPG_EINVAL, "compgt mode %d", mode);
-    }
-    if (rc == NBPG_OK) {
-        uint32_t m = (uint32_t)(slices * k), n2 = 1;
-        while (n2 < m)
-            n2 <<= 1;
-        hipLaunchKernelGGL(compgt_merge_kernel, dim3((uint32_t)nq), dim3(256), n2 * 8, s, partial,
-                           (uint32_t)slices, k, d_out_ids, d_out_dists);
-        if (hipGetLastError() != hipSuccess)
-            rc = set_error(NBPG_EHIP, "compgt merge launch");
-    }
-    hipFreeAsync(partial, s);
-    return rc;
+        return compgt_mfma(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq, dims,
+                           k, d_out_ids, d_out_dists, s);
+    if (mode == NBPG_GT_PLAIN)
+        return compgt_plain_launch(dtype, (const uint8_t *)d_base, n, dims,
+                                   (const uint8_t *)d_queries, nq, dims, k, d_out_ids,
+                                   d_out_dists, s);
+    return set_error(NBPG_EINVAL, "compgt mode %d", mode);
 }

@@ -1,16 +1,323 @@
-// compgt_mfma.hip -- int8 MFMA form of the ground-truth scan (placeholder
-// until the MFMA tile kernel lands).
+// compgt_mfma.hip -- exact brute-force top-k on int8 matrix cores.
+//
+// Same result as compgt_plain (the exact (dist, id) top-k of every query,
+// what compute_groundtruth at CompGT.cpp:29-134 aims at), three passes:
+//   1. threshold: the plain scan over S base rows sampled at a fixed stride
+//      gives each query's k-th smallest distance there.  The k-th smallest
+//      distance of a subset is never below the global k-th smallest, so every
+//      true top-k row has a distance <= this threshold.
+//   2. scan: v_mfma_i32_16x16x64_i8 Gram tiles.  u8 rows enter as x - 128
+//      (x ^ 0x80 per byte); |x-q|^2 = |x'-q'|^2 = |x'|^2 + |q'|^2 - 2<x',q'>
+//      is exact in int32.  A wave keeps 16*QW queries' fragments in
+//      registers and streams 16-row base blocks; every row within its
+//      query's threshold is appended to that query's candidate list.
+//   3. select: per query, a bitonic sort of its candidates; a query whose
+//      list overflowed is recomputed by the plain kernel.
+#include <algorithm>
+#include <vector>
+
 #include "internal.h"
 
 namespace nbpg {
 
-int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                uint32_t dims, uint32_t k, uint64_t *partial, uint32_t slices, uint64_t slice_rows,
-                hipStream_t stream)
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+#define MF_CAND 4096 // candidate slots per query
+#define MF_QW 2      // 16-query A tiles per wave
+#define MF_WAVES 4
+
+int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                        const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
+                        uint32_t *ids, float *dists, hipStream_t s);
+
+// |x'|^2 per row, x' = x ^ 0x80 (u8) or x (i8), over dims bytes
+__global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, uint32_t dims,
+                                int is_u8, uint32_t *__restrict__ out)
 {
-    (void)dtype; (void)base; (void)n; (void)queries; (void)nq; (void)dims; (void)k;
-    (void)partial; (void)slices; (void)slice_rows; (void)stream;
-    return set_error(NBPG_EUNSUP, "MFMA ground truth not built yet");
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(rows + i * dims);
+    int acc = 0;
+    for (uint32_t t = 0; t < dims / 4; ++t) {
+        const uint32_t v = is_u8 ? (w[t] ^ 0x80808080u) : w[t];
+        acc = __builtin_amdgcn_sdot4((int)v, (int)v, acc, false);
+    }
+    out[i] = (uint32_t)acc;
+}
+
+// threshold key: the sample's k-th distance with the largest id, so every
+// row at that distance or closer passes `key <= thr`
+__global__ void mf_thresholds_kernel(const float *__restrict__ d, uint64_t nq, uint32_t k,
+                                     uint64_t *__restrict__ thr, uint32_t *__restrict__ cnt,
+                                     uint32_t *__restrict__ ovf)
+{
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    thr[q] = make_key(d[q * k + k - 1], 0xFFFFFFFFu);
+    cnt[q] = 0;
+    ovf[q] = 0;
+}
+
+// rows[i] = src[idx[i]] (dims bytes each)
+__global__ void mf_gather_kernel(const uint8_t *__restrict__ src, const uint32_t *__restrict__ idx,
+                                 uint32_t m, uint32_t dims, uint8_t *__restrict__ rows)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)m * dims)
+        return;
+    const uint64_t i = t / dims, b = t % dims;
+    rows[t] = src[(uint64_t)idx[i] * dims + b];
+}
+
+// out[idx[i]] = compact[i] (k results each)
+__global__ void mf_scatter_kernel(const uint32_t *__restrict__ cids, const float *__restrict__ cd,
+                                  const uint32_t *__restrict__ idx, uint32_t m, uint32_t k,
+                                  uint32_t *__restrict__ ids, float *__restrict__ dists)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)m * k)
+        return;
+    const uint64_t i = t / k, j = t % k, o = (uint64_t)idx[i] * k + j;
+    ids[o] = cids[t];
+    if (dists)
+        dists[o] = cd[t];
+}
+
+template <int KS>
+__global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
+    const uint8_t *__restrict__ base, uint64_t n, const uint8_t *__restrict__ queries, uint64_t nq,
+    uint32_t dims, int is_u8, const uint32_t *__restrict__ nx, const uint32_t *__restrict__ nqv,
+    const uint64_t *__restrict__ thr, uint64_t slice_rows, uint32_t *__restrict__ cnt,
+    uint64_t *__restrict__ cand)
+{
+    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const uint32_t row16 = lane % 16, kq = lane / 16;
+    const uint64_t q0 = ((uint64_t)blockIdx.x * MF_WAVES + wv) * (16 * MF_QW);
+    if (q0 >= nq)
+        return;
+    const uint32_t xmask = is_u8 ? 0x80808080u : 0u;
+    // query fragments: lane supplies 16 bytes of query row (l % 16) at k-chunk (l / 16)
+    v4i a[MF_QW][KS];
+    uint64_t th[MF_QW][4]; // 0 = no query (keys are compared with <)
+    uint32_t qn[MF_QW][4];
+#pragma unroll
+    for (int t = 0; t < MF_QW; ++t) {
+        const uint64_t q = q0 + 16 * t + row16;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const uint32_t off = 64 * s + 16 * kq;
+            v4i v = {0, 0, 0, 0};
+            if (q < nq && off < dims) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(queries + q * dims + off);
+                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
+            }
+            a[t][s] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint64_t qr = q0 + 16 * t + 4 * kq + r;
+            th[t][r] = qr < nq ? thr[qr] + 1 : 0ull; // thr < ~0: finite distance
+            qn[t][r] = qr < nq ? nqv[qr] : 0u;
+        }
+    }
+    const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows;
+    const uint64_t r1 = min(n, r0 + slice_rows);
+    for (uint64_t b0 = r0; b0 < r1; b0 += 16) {
+        const uint64_t row = b0 + row16;
+        const bool rv = row < r1;
+        v4i b[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const uint32_t off = 64 * s + 16 * kq;
+            v4i v = {0, 0, 0, 0};
+            if (rv && off < dims) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(base + row * dims + off);
+                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
+            }
+            b[s] = v;
+        }
+        const uint32_t xn = rv ? nx[row] : 0u;
+#pragma unroll
+        for (int t = 0; t < MF_QW; ++t) {
+            v4i c = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+                c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t d = qn[t][r] + xn - 2u * (uint32_t)c[r];
+                const uint64_t key = make_key((float)(int32_t)d, (uint32_t)row);
+                if (rv && key < th[t][r]) {
+                    const uint64_t qr = q0 + 16 * t + 4 * kq + r;
+                    const uint32_t p = atomicAdd(&cnt[qr], 1u);
+                    if (p < MF_CAND)
+                        cand[qr * MF_CAND + p] = key;
+                }
+            }
+        }
+    }
+}
+
+// per query: sort the candidates, write k; flag overflowed queries
+__global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restrict__ cand,
+                                                       const uint32_t *__restrict__ cnt, uint32_t k,
+                                                       uint32_t *__restrict__ ids,
+                                                       float *__restrict__ dists,
+                                                       uint32_t *__restrict__ overflow)
+{
+    __shared__ uint64_t s[MF_CAND];
+    const uint64_t q = blockIdx.x;
+    const uint32_t m = cnt[q];
+    if (m > MF_CAND) {
+        if (threadIdx.x == 0)
+            overflow[q] = 1;
+        return;
+    }
+    uint32_t n2 = 1;
+    while (n2 < m)
+        n2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
+        s[i] = i < m ? cand[q * MF_CAND + i] : ~0ull;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= n2; kk <<= 1)
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const uint64_t x = s[i], y = s[p];
+                    if ((x > y) == ((i & kk) == 0)) {
+                        s[i] = y;
+                        s[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+        ids[q * k + i] = key_id(s[i]);
+        if (dists)
+            dists[q * k + i] = key_dist(s[i]);
+    }
+}
+
+int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
+                uint32_t dims, uint32_t k, uint32_t *out_ids, float *out_dists, hipStream_t s)
+{
+    if (dtype != NBPG_U8 && dtype != NBPG_I8)
+        return set_error(NBPG_EUNSUP, "MFMA ground truth takes u8 or i8 rows");
+    if (dims % 16 || dims > 256 || (reinterpret_cast<uintptr_t>(base) & 15) ||
+        (reinterpret_cast<uintptr_t>(queries) & 15))
+        return set_error(NBPG_EUNSUP, "MFMA ground truth needs 16-byte rows (dims %u)", dims);
+    const int is_u8 = dtype == NBPG_U8;
+    const uint32_t KS = (dims + 63) / 64;
+    // 1. thresholds from every step-th row: about k * n / S rows then pass
+    //    per query, kept to a quarter of the candidate slots
+    const uint64_t want = std::max<uint64_t>(
+        std::max<uint64_t>(131072, 64ull * k), (4ull * k * n + MF_CAND - 1) / MF_CAND);
+    const uint64_t step = std::max<uint64_t>(1, n / want);
+    const uint64_t S = (n + step - 1) / step; // >= min(n, want) >= k
+    uint32_t *tid = nullptr, *cnt = nullptr, *nx = nullptr, *nqv = nullptr, *ovf = nullptr;
+    float *td = nullptr;
+    uint64_t *thr = nullptr, *cand = nullptr;
+    auto cleanup = [&]() {
+        hipFreeAsync(tid, s);
+        hipFreeAsync(td, s);
+        hipFreeAsync(cnt, s);
+        hipFreeAsync(nx, s);
+        hipFreeAsync(nqv, s);
+        hipFreeAsync(ovf, s);
+        hipFreeAsync(thr, s);
+        hipFreeAsync(cand, s);
+    };
+    bool ok = hipMallocAsync((void **)&tid, nq * k * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&td, nq * k * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&cnt, nq * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&nx, n * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&nqv, nq * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&ovf, nq * 4, s) == hipSuccess &&
+              hipMallocAsync((void **)&thr, nq * 8, s) == hipSuccess &&
+              hipMallocAsync((void **)&cand, nq * MF_CAND * 8, s) == hipSuccess;
+    if (!ok) {
+        cleanup();
+        return set_error(NBPG_ENOMEM, "MFMA ground-truth workspace");
+    }
+    int rc = compgt_plain_launch(dtype, base, S, step * dims, queries, nq, dims, k, tid, td, s);
+    if (rc != NBPG_OK) {
+        cleanup();
+        return rc;
+    }
+    hipLaunchKernelGGL(mf_thresholds_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
+                       td, nq, k, thr, cnt, ovf);
+    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, base, n,
+                       dims, is_u8, nx);
+    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
+                       queries, nq, dims, is_u8, nqv);
+    // 2. MFMA scan: query tiles x base slices
+    const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
+    const uint64_t tiles = (nq + qpb - 1) / qpb;
+    uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / tiles);
+    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 4096));
+    const uint64_t slice_rows = ((n + slices - 1) / slices + 15) / 16 * 16;
+    slices = (n + slice_rows - 1) / slice_rows;
+    dim3 grid((uint32_t)tiles, (uint32_t)slices), block(64 * MF_WAVES);
+    switch (KS) {
+    case 1: hipLaunchKernelGGL((mf_scan_kernel<1>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
+    case 2: hipLaunchKernelGGL((mf_scan_kernel<2>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
+    case 3: hipLaunchKernelGGL((mf_scan_kernel<3>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
+    default: hipLaunchKernelGGL((mf_scan_kernel<4>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
+    }
+    if (hipGetLastError() != hipSuccess) {
+        cleanup();
+        return set_error(NBPG_EHIP, "MFMA scan launch");
+    }
+    // 3. select, then redo overflowed queries with the plain scan
+    hipLaunchKernelGGL(mf_select_kernel, dim3((uint32_t)nq), dim3(256), 0, s, cand, cnt, k, out_ids,
+                       out_dists, ovf);
+    std::vector<uint32_t> h(nq);
+    if (hipMemcpyAsync(h.data(), ovf, nq * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        return set_error(NBPG_EHIP, "overflow readback");
+    }
+    std::vector<uint32_t> redo;
+    for (uint64_t q = 0; q < nq; ++q)
+        if (h[q])
+            redo.push_back((uint32_t)q);
+    if (!redo.empty()) {
+        const uint32_t m = (uint32_t)redo.size();
+        uint32_t *idx = nullptr, *cids = nullptr;
+        float *cd = nullptr;
+        uint8_t *rows = nullptr;
+        ok = hipMallocAsync((void **)&idx, m * 4ull, s) == hipSuccess &&
+             hipMallocAsync((void **)&cids, (uint64_t)m * k * 4, s) == hipSuccess &&
+             hipMallocAsync((void **)&cd, (uint64_t)m * k * 4, s) == hipSuccess &&
+             hipMallocAsync((void **)&rows, (uint64_t)m * dims, s) == hipSuccess &&
+             hipMemcpyAsync(idx, redo.data(), m * 4ull, hipMemcpyHostToDevice, s) == hipSuccess;
+        if (ok) {
+            const uint64_t tb = (uint64_t)m * dims, tk = (uint64_t)m * k;
+            hipLaunchKernelGGL(mf_gather_kernel, dim3((uint32_t)((tb + 255) / 256)), dim3(256), 0,
+                               s, queries, idx, m, dims, rows);
+            rc = compgt_plain_launch(dtype, base, n, dims, rows, m, dims, k, cids, cd, s);
+            if (rc == NBPG_OK)
+                hipLaunchKernelGGL(mf_scatter_kernel, dim3((uint32_t)((tk + 255) / 256)),
+                                   dim3(256), 0, s, cids, cd, idx, m, k, out_ids, out_dists);
+            // the host vector redo must outlive the copy
+            if (hipStreamSynchronize(s) != hipSuccess && rc == NBPG_OK)
+                rc = set_error(NBPG_EHIP, "overflow redo");
+        } else {
+            rc = set_error(NBPG_ENOMEM, "overflow redo workspace");
+        }
+        hipFreeAsync(idx, s);
+        hipFreeAsync(cids, s);
+        hipFreeAsync(cd, s);
+        hipFreeAsync(rows, s);
+        if (!ok)
+            hipStreamSynchronize(s);
+    }
+    cleanup();
+    return rc;
 }
 
 } // namespace nbpg

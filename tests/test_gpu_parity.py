@@ -196,7 +196,7 @@ def test_prune_matches_oracle(gpu, small, alpha, cosa, R):
     ix.close()
 
 
-@pytest.mark.parametrize("mode", [0])
+@pytest.mark.parametrize("mode", [0, 1])
 def test_compgt_exact(gpu, mode):
     N = gpu
     base, q = D.workload_small(20000, 100)
@@ -219,6 +219,40 @@ def test_compgt_i8_and_4bit(gpu):
         oi, od = O.compgt(base, q, 50, dtype, tie_mode=1)
         assert (ids.cpu().numpy().view(np.uint32) == oi).all(), dtype
         assert (d.cpu().numpy() == od).all(), dtype
+
+
+def test_compgt_mfma_shapes_and_overflow(gpu):
+    """MFMA ground truth equals the oracle for i8 / u8 rows of 16..256 bytes,
+    ragged n and nq, and when many rows tie so that the candidate lists
+    overflow and the plain scan redoes those queries."""
+    N = gpu
+    rng = np.random.default_rng(5)
+    for dtype, dims, n, nq, k in [(N.I8, 64, 5003, 37, 20), (N.U8, 256, 3001, 130, 256),
+                                  (N.U8, 16, 20011, 200, 7), (N.I8, 192, 4099, 65, 33)]:
+        base = rng.integers(0, 256, (n, dims), dtype=np.uint8)
+        q = rng.integers(0, 256, (nq, dims), dtype=np.uint8)
+        ids, d = N.compgt(N.to_device(base), N.to_device(q), k, dtype, N.GT_MFMA)
+        oi, od = O.compgt(base, q, k, dtype, tie_mode=1)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), (dtype, dims)
+        assert (d.cpu().numpy() == od).all(), (dtype, dims)
+    # 6000 copies of 3 rows: every query's threshold admits ~2000 rows
+    base = np.repeat(rng.integers(0, 256, (3, 128), dtype=np.uint8), 6000, axis=0)
+    q = np.concatenate([base[:4], rng.integers(0, 256, (20, 128), dtype=np.uint8)])
+    ids, d = N.compgt(N.to_device(base), N.to_device(q), 100, N.U8, N.GT_MFMA)
+    oi, od = O.compgt(base, q, 100, N.U8, tie_mode=1)
+    assert (ids.cpu().numpy().view(np.uint32) == oi).all()
+    assert (d.cpu().numpy() == od).all()
+    # base sorted by distance to a point: the strided sample still bounds the
+    # threshold from above
+    base = rng.integers(0, 256, (400000, 32), dtype=np.uint8)
+    base = base[np.argsort(((base.astype(np.int64) - 128) ** 2).sum(1), kind="stable")]
+    q = rng.integers(100, 156, (50, 32), dtype=np.uint8)
+    ids, d = N.compgt(N.to_device(base), N.to_device(q), 10, N.U8, N.GT_MFMA)
+    oi, od = O.compgt(base, q, 10, N.U8, tie_mode=1)
+    assert (ids.cpu().numpy().view(np.uint32) == oi).all()
+    with pytest.raises(N.NbpgError):
+        N.compgt(N.to_device(base[:, :28].copy()), N.to_device(q[:, :28].copy()), 10, N.U8,
+                 N.GT_MFMA)
 
 
 def live(rows):
