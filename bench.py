@@ -122,7 +122,16 @@ def main():
             and np.array_equal(gtd, p_d.cpu().numpy())):
         raise SystemExit("MFMA and VALU ground truth differ")
     del p_ids, p_d
-    log("device ground truth top-100: MFMA %.3fs, VALU %.3fs (identical)" % (gt_s, gt_plain_s))
+    # the reference's own slot-replacement rule (what its CompGT writes)
+    t = time.time()
+    r_ids, _ = N.compgt(base_d, q_gt, 100, N.U8, N.GT_MFMA | N.GT_REFTIES)
+    torch.cuda.synchronize()
+    gt_ref_s = time.time() - t
+    gt_ref = r_ids.cpu().numpy().view(np.uint32)
+    del r_ids
+    gt_ref_differs = int((gt_ref != gt).any(axis=1).sum())
+    log("device ground truth top-100: MFMA %.3fs, VALU %.3fs (identical); reference rule %.3fs,"
+        " %d queries differ from the exact top-100" % (gt_s, gt_plain_s, gt_ref_s, gt_ref_differs))
 
     def recall_at(ids, g):
         return AT.recall_with_ties(ids, g, gtd[:len(g)], 10)
@@ -241,6 +250,8 @@ def main():
         "build_s": round(build_s, 2),
         "groundtruth_s": round(gt_s, 3),
         "groundtruth_valu_s": round(gt_plain_s, 3),
+        "groundtruth_refrule_s": round(gt_ref_s, 3),
+        "groundtruth_refrule_queries_differing": gt_ref_differs,
         "mean_visited": round(float(vis.mean()), 1),
         "mean_dist_cmps": round(float(cmps.mean()), 1),
         "autotune": [{"config": list(r["config"]), "L": r["L"], "qps": round(r["qps"], 1)}

@@ -43,6 +43,12 @@ extern "C" {
 /* ---- CompGT modes ---- */
 #define NBPG_GT_PLAIN 0 /* VALU dot4, one query tile per workgroup */
 #define NBPG_GT_MFMA 1  /* int8 MFMA Gram tiles, exact int32 */
+/* OR into the mode: keep the reference's slot-replacement rule
+ * (CompGT.cpp:63-97) instead of the exact (dist, id) top-k.  That rule
+ * decides which of several rows tied at the k-th distance survive, and its
+ * stale-maximum step (CompGT.cpp:82-84) can evict a row of the true top-k;
+ * this mode reproduces both. */
+#define NBPG_GT_REFTIES 0x100
 
 /* (id, dist) record of the reference (query_params.h:97-104), 8 bytes */
 typedef struct {
@@ -145,7 +151,8 @@ int nbpg_robust_prune_batch(const nbpg_index *idx, const uint32_t *d_nodes, uint
                             void *stream);
 
 /* ---- brute-force ground truth: compute_groundtruth (CompGT.cpp:29-134)
- * exact (dist, id) top-k per query, rows packed (dims bytes each) */
+ * per query the k rows sorted by (dist, id), rows packed (dims bytes each).
+ * mode = NBPG_GT_PLAIN or NBPG_GT_MFMA, optionally | NBPG_GT_REFTIES. */
 int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries, uint64_t nq,
                 uint32_t dims, uint32_t k, int mode, uint32_t *d_out_ids, float *d_out_dists,
                 void *stream);

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libnbpg_amd.so")
 
 U8, I8, U4X2, I4X2 = 0, 1, 2, 3
 SEARCH_REF, SEARCH_FAST = 0, 1
-GT_PLAIN, GT_MFMA = 0, 1
+GT_PLAIN, GT_MFMA, GT_REFTIES = 0, 1, 0x100
 SENT = 0xFFFFFFFF
 DTYPE_OF_EXT = {".u8bin": U8, ".i8bin": I8, ".u4bin": U4X2, ".i4bin": I4X2}
 

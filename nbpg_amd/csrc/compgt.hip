@@ -211,6 +211,10 @@ __global__ __launch_bounds__(256) void compgt_merge_kernel(const uint64_t *__res
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
                 uint32_t dims, uint32_t k, uint32_t *out_ids, float *out_dists, hipStream_t stream);
 
+int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
+                   uint32_t dims, uint32_t k, bool use_mfma, uint32_t *ids, float *dists,
+                   hipStream_t s);
+
 // plain scan + slice merge into ids / dists (dists may be NULL).  Base row r
 // is at base + r * row_stride and is reported as id r.
 int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
@@ -281,6 +285,17 @@ extern "C" int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void
     if (nq == 0)
         return NBPG_OK;
     hipStream_t s = as_stream(stream);
+    if (mode & NBPG_GT_REFTIES) {
+        const int engine = mode & ~NBPG_GT_REFTIES;
+        if (engine == NBPG_GT_MFMA &&
+            ((dtype != NBPG_U8 && dtype != NBPG_I8) || dims % 16 ||
+             ((reinterpret_cast<uintptr_t>(d_base) | reinterpret_cast<uintptr_t>(d_queries)) & 15)))
+            return set_error(NBPG_EUNSUP, "MFMA ground truth needs 16-byte u8 / i8 rows");
+        if (engine != NBPG_GT_MFMA && engine != NBPG_GT_PLAIN)
+            return set_error(NBPG_EINVAL, "compgt mode %d", mode);
+        return compgt_refrule(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq,
+                              dims, k, engine == NBPG_GT_MFMA, d_out_ids, d_out_dists, s);
+    }
     if (mode == NBPG_GT_MFMA)
         return compgt_mfma(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq, dims,
                            k, d_out_ids, d_out_dists, s);

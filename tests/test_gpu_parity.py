@@ -255,6 +255,38 @@ def test_compgt_mfma_shapes_and_overflow(gpu):
                  N.GT_MFMA)
 
 
+@pytest.mark.parametrize("engine", [0, 1])
+def test_compgt_reference_rule(gpu, golden, engine):
+    """NBPG_GT_REFTIES replays CompGT.cpp:63-97: the compiled reference's own
+    output on the golden set, the hand traces of its stale-maximum step, and
+    the oracle on tie-heavy and random inputs."""
+    N = gpu
+    mode = engine | N.GT_REFTIES
+    ids, d = N.compgt(N.to_device(golden["base"]), N.to_device(golden["queries"]), 50, N.U8, mode)
+    assert (ids.cpu().numpy().view(np.uint32) == golden["gt_ids"]).all()
+    assert (d.cpu().numpy() == golden["gt_dists"]).all()
+    from test_oracle import stale_rule_cases
+    for base, k, ids_want, d_want in stale_rule_cases():
+        base16 = np.zeros((len(base), 16), np.uint8)
+        base16[:, :4] = base
+        ids, d = N.compgt(N.to_device(base16), N.to_device(np.zeros((1, 16), np.uint8)), k, N.U8,
+                          mode)
+        assert ids.cpu().numpy().view(np.uint32)[0].tolist() == ids_want
+        assert d.cpu().numpy()[0].tolist() == d_want
+    rng = np.random.default_rng(21)
+    cases = [(N.U8, 16, 3000, 70, 25, 4), (N.U8, 128, 20000, 100, 100, 256),
+             (N.I8, 64, 9001, 33, 200, 256), (N.U8, 32, 5000, 40, 1, 3)]
+    if engine == 0:
+        cases += [(N.U8, 12, 4000, 20, 10, 3), (N.U4X2, 32, 6000, 25, 50, 256)]
+    for dtype, dims, n, nq, k, hi in cases:
+        base = rng.integers(0, hi, (n, dims), dtype=np.uint8)
+        q = rng.integers(0, hi, (nq, dims), dtype=np.uint8)
+        ids, d = N.compgt(N.to_device(base), N.to_device(q), k, dtype, mode)
+        oi, od = O.compgt(base, q, k, dtype, tie_mode=0)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), (dtype, dims, k)
+        assert (d.cpu().numpy() == od).all(), (dtype, dims, k)
+
+
 def live(rows):
     out = rows.copy()
     R = rows.shape[1] - 1

@@ -86,6 +86,31 @@ def test_compgt_reference_rule_golden(golden):
     assert (d == golden["gt_dists"]).all()
 
 
+def stale_rule_cases():
+    """Hand-worked traces of CompGT.cpp:63-97 for a zero query and rows whose
+    squared norms are the listed distances.
+    [9, 2, 3, 1, 5], k=3: fill -> max 9 in slot 0; row 3 (d=1) replaces slot
+    0, the rescan starts from the pre-write 9 so the max stays 9 at slot 0;
+    row 4 (d=5) then evicts slot 0 (the 1), rescan from 1 finds 3 at slot 2.
+    Slots {5, 2, 3} -> ids [1, 2, 4], although row 3 is the nearest.
+    [5, 5, 3], k=2: row 2 replaces slot 0 (first max), ids [2, 1]; the exact
+    (dist, id) rule would keep [2, 0]."""
+    rows = {9: [3, 0, 0, 0], 2: [1, 1, 0, 0], 3: [1, 1, 1, 0], 1: [1, 0, 0, 0],
+            5: [2, 1, 0, 0]}
+    return [
+        (np.array([rows[d] for d in [9, 2, 3, 1, 5]], np.uint8), 3, [1, 2, 4], [2, 3, 5]),
+        (np.array([rows[d] for d in [5, 5, 3]], np.uint8), 2, [2, 1], [3, 5]),
+    ]
+
+
+def test_compgt_reference_rule_hand_traces():
+    for base, k, ids_want, d_want in stale_rule_cases():
+        q = np.zeros((1, 4), np.uint8)
+        ids, d = O.compgt(base, q, k, O.U8, tie_mode=0)
+        assert ids[0].tolist() == ids_want
+        assert d[0].tolist() == d_want
+
+
 def test_compgt_lexicographic_brute_force():
     rng = np.random.default_rng(3)
     base = rng.integers(0, 4, (300, 16), dtype=np.uint8)  # many exact ties
