@@ -35,6 +35,7 @@ extern "C" {
 #define NBPG_I8 1
 #define NBPG_U4X2 2 /* .u4bin: two 4-bit codes per byte */
 #define NBPG_I4X2 3 /* .i4bin: distance identical to U4X2 (point.h:107-120) */
+#define NBPG_F32 4  /* .fbin / .bin: PCA input rows only (matrix.h:27-57) */
 
 /* ---- search modes ---- */
 #define NBPG_SEARCH_REF 0  /* one query per workgroup, reference-exact IDs */
@@ -49,6 +50,18 @@ extern "C" {
  * stale-maximum step (CompGT.cpp:82-84) can evict a row of the true top-k;
  * this mode reproduces both. */
 #define NBPG_GT_REFTIES 0x100
+
+/* ---- PCA: projection forms and code types.  The code type is what the
+ *      reference derives from the output extension (PCACompress.cpp:25-37):
+ *      .fbin = F32 (no codes), .u8bin, .i8bin (chunk 1), .u4bin, .i4bin
+ *      (chunk 2, two codes per byte). ---- */
+#define NBPG_PCA_PLAIN 0 /* fp32, products summed in index order: y bit-identical */
+#define NBPG_PCA_BF16 1  /* bf16 MFMA, fp32 accumulation, quantizer fused */
+#define NBPG_CODE_F32 0
+#define NBPG_CODE_U8 1
+#define NBPG_CODE_I8 2
+#define NBPG_CODE_U4 3
+#define NBPG_CODE_I4 4
 
 /* (id, dist) record of the reference (query_params.h:97-104), 8 bytes */
 typedef struct {
@@ -156,6 +169,28 @@ int nbpg_robust_prune_batch(const nbpg_index *idx, const uint32_t *d_nodes, uint
 int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries, uint64_t nq,
                 uint32_t dims, uint32_t k, int mode, uint32_t *d_out_ids, float *d_out_dists,
                 void *stream);
+
+/* ---- PCACompress (PCACompress.cpp:21-79): y = bias + W x per row
+ * (matmul, matrix.h:86-97) and its codes (compress, matrix.h:99-127).
+ * x: n rows of d_in elements of x_dtype (NBPG_F32, NBPG_U8 or NBPG_I8; the
+ * byte types become floats as loadFloatPoint does, matrix.h:27-57).
+ * W: d_out x d_in f32 row-major and bias: d_out f32, as loadMatrix reads a
+ * .mat file (matrix.h:59-83; zeros when it has no bias).
+ * code != NBPG_CODE_F32: d_out_codes gets n x (d_out / chunk) bytes;
+ * d_out_y (optional there, required for NBPG_CODE_F32) gets y, n x d_out.
+ * NBPG_PCA_BF16 needs d_in % 8 == 0, d_out a multiple of 16 up to 128. */
+int nbpg_pca_project(int x_dtype, const void *d_x, uint64_t n, uint32_t d_in, const float *d_W,
+                     const float *d_bias, uint32_t d_out, int mode, int code,
+                     uint8_t *d_out_codes, float *d_out_y, void *stream);
+/* compress (matrix.h:99-127) alone: y n x d f32 -> n x (d / chunk) codes */
+int nbpg_pca_quantize(const float *d_y, uint64_t n, uint32_t d, int code, uint8_t *d_out,
+                      void *stream);
+/* FP64 statistics for a PCA fit (the reference's fit script,
+ * py_data_tools/pca_process.py named at README.md:43, is not in its tree):
+ * mean[d] and, if d_cov is non-NULL, the covariance d x d
+ * sum (x - mean)(x - mean)^T / (n - 1), accumulated in FP64. */
+int nbpg_pca_stats_f64(int x_dtype, const void *d_x, uint64_t n, uint32_t d, double *d_mean,
+                       double *d_cov, void *stream);
 
 /* ---- graph build: build_vamana (nsw.h:547-663) on the device; the
  * index's points must be set; the graph is (re)built in place (set a graph

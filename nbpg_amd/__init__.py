@@ -14,7 +14,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnbpg_amd.so")
 
-U8, I8, U4X2, I4X2 = 0, 1, 2, 3
+U8, I8, U4X2, I4X2, F32 = 0, 1, 2, 3, 4
+PCA_PLAIN, PCA_BF16 = 0, 1
+CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
+# PCACompress picks the code type from the output extension (PCACompress.cpp:25-37)
+CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
+               ".i4bin": CODE_I4}
 SEARCH_REF, SEARCH_FAST = 0, 1
 GT_PLAIN, GT_MFMA, GT_REFTIES = 0, 1, 0x100
 SENT = 0xFFFFFFFF
@@ -69,6 +74,9 @@ _SIGS = {
     "nbpg_robust_prune_batch": (I32, [VP, VP, U64, VP, VP, U32, VP, DBL, DBL, VP, VP, VP, VP]),
     "nbpg_compgt": (I32, [I32, VP, U64, VP, U64, U32, U32, I32, VP, VP, VP]),
     "nbpg_build": (I32, [VP, C.POINTER(BuildParams), C.POINTER(U64)]),
+    "nbpg_pca_project": (I32, [I32, VP, U64, U32, VP, VP, U32, I32, I32, VP, VP, VP]),
+    "nbpg_pca_quantize": (I32, [VP, U64, U32, I32, VP, VP]),
+    "nbpg_pca_stats_f64": (I32, [I32, VP, U64, U32, VP, VP, VP]),
 }
 
 
@@ -237,6 +245,55 @@ def compgt(base_dev, q_dev, k, dtype=U8, mode=GT_PLAIN):
                              base_dev.shape[1], k, mode, _ptr(ids), _ptr(d), _stream()),
            "nbpg_compgt")
     return ids, d
+
+
+def _x_dtype(t):
+    torch = _torch()
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.uint8:
+        return U8
+    if t.dtype == torch.int8:
+        return I8
+    raise NbpgError("PCA input rows must be float32, uint8 or int8, not %s" % t.dtype)
+
+
+def pca_project(x_dev, W_dev, bias_dev, code=CODE_I8, mode=PCA_BF16, want_y=False):
+    """PCACompress's projection + quantizer over device rows x (n x d_in).
+    Returns (codes or None, y or None)."""
+    torch = _torch()
+    n, d_in = x_dev.shape
+    d_out = W_dev.shape[0]
+    chunk = 2 if code in (CODE_U4, CODE_I4) else 1
+    codes = None if code == CODE_F32 else torch.empty((n, d_out // chunk), dtype=torch.uint8,
+                                                      device=x_dev.device)
+    y = torch.empty((n, d_out), dtype=torch.float32, device=x_dev.device) \
+        if (want_y or code == CODE_F32) else None
+    _check(lib().nbpg_pca_project(_x_dtype(x_dev), _ptr(x_dev), n, d_in, _ptr(W_dev),
+                                  _ptr(bias_dev), d_out, mode, code, _ptr(codes), _ptr(y),
+                                  _stream()), "nbpg_pca_project")
+    return codes, y
+
+
+def pca_quantize(y_dev, code):
+    torch = _torch()
+    n, d = y_dev.shape
+    chunk = 2 if code in (CODE_U4, CODE_I4) else 1
+    out = torch.empty((n, d // chunk), dtype=torch.uint8, device=y_dev.device)
+    _check(lib().nbpg_pca_quantize(_ptr(y_dev), n, d, code, _ptr(out), _stream()),
+           "nbpg_pca_quantize")
+    return out
+
+
+def pca_stats(x_dev, cov=True):
+    """FP64 (mean, covariance) of device rows x."""
+    torch = _torch()
+    n, d = x_dev.shape
+    mean = torch.empty(d, dtype=torch.float64, device=x_dev.device)
+    c = torch.empty((d, d), dtype=torch.float64, device=x_dev.device) if cov else None
+    _check(lib().nbpg_pca_stats_f64(_x_dtype(x_dev), _ptr(x_dev), n, d, _ptr(mean), _ptr(c),
+                                    _stream()), "nbpg_pca_stats_f64")
+    return mean, c
 
 
 def pack_cands(ids, dists):

@@ -1,0 +1,145 @@
+"""PCACompress on the device against the reference's own outputs, the oracle
+and an FP64 projection."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_pca.npz")
+CODES = [(1, 0, 1, "u8bin"), (2, 1, 1, "i8bin"), (3, 0, 2, "u4bin"), (4, 1, 2, "i4bin")]
+
+
+def test_plain_projection_matches_reference_golden(gpu):
+    """NBPG_PCA_PLAIN: y bit-identical to the reference's PCACompress .fbin
+    output, codes identical for every code type."""
+    N = gpu
+    import torch
+    g = np.load(GOLD)
+    for t in "fg":
+        x = torch.from_numpy(g["x_" + t]).cuda()
+        W = torch.from_numpy(g["W_" + t]).cuda()
+        b = torch.from_numpy(g["b_" + t]).cuda()
+        _, y = N.pca_project(x, W, b, code=N.CODE_F32, mode=N.PCA_PLAIN)
+        assert (y.cpu().numpy() == g["out_%s_fbin" % t]).all(), t
+        for code, _, _, ext in CODES:
+            c, _ = N.pca_project(x, W, b, code=code, mode=N.PCA_PLAIN)
+            assert (c.cpu().numpy() == g["out_%s_%s" % (t, ext)]).all(), (t, ext)
+
+
+def test_quantizer_bit_exact(gpu):
+    """compress (matrix.h:99-127) on boundary values: -1, 1, just outside,
+    exact (v+1)*127.5 integers, huge, infinite and NaN inputs."""
+    N = gpu
+    import torch
+    rng = np.random.default_rng(3)
+    y = (rng.standard_normal((257, 64)) * 0.8).astype(np.float32)
+    special = np.array([-1.0, 1.0, -1.0000001, 1.0000001, 0.0, -0.0, 3e9, -3e9, np.inf, -np.inf,
+                        np.nan, 2 / 255 - 1, 4 / 255 - 1, 1 - 1 / 255, 127 / 127.5 - 1,
+                        128 / 127.5 - 1], np.float32)
+    y[0, :len(special)] = special
+    y[1, :] = np.linspace(-1.01, 1.01, 64, dtype=np.float32)
+    yd = torch.from_numpy(y).cuda()
+    for code, sg, ch, _ in CODES:
+        got = N.pca_quantize(yd, code).cpu().numpy()
+        assert (got == O.compress(y, sg, ch)).all(), code
+
+
+@pytest.mark.parametrize("xt", ["f32", "u8", "i8"])
+def test_plain_projection_matches_oracle(gpu, xt):
+    N = gpu
+    import torch
+    rng = np.random.default_rng(11)
+    n, d_in, d_out = 1500, 200, 24
+    if xt == "f32":
+        x = (rng.standard_normal((n, d_in)) * 0.1).astype(np.float32)
+        xf = x
+    else:
+        x = rng.integers(0, 256, (n, d_in), dtype=np.uint8)
+        if xt == "i8":
+            x = x.view(np.int8)
+        xf = x.astype(np.float32)
+    W = (rng.standard_normal((d_out, d_in)) / (60.0 if xt == "f32" else 6000.0)).astype(np.float32)
+    b = (rng.standard_normal(d_out) * 0.1).astype(np.float32)
+    c, y = N.pca_project(torch.from_numpy(x).cuda(), torch.from_numpy(W).cuda(),
+                         torch.from_numpy(b).cuda(), code=N.CODE_I4, mode=N.PCA_PLAIN,
+                         want_y=True)
+    yo = O.matmul(xf, W, b)
+    assert (y.cpu().numpy() == yo).all()
+    assert (c.cpu().numpy() == O.compress(yo, 1, 2)).all()
+
+
+@pytest.mark.parametrize("xt,d_in,d_out", [("f32", 960, 64), ("f32", 136, 16), ("u8", 128, 32),
+                                           ("i8", 96, 128)])
+def test_bf16_projection_error_bound(gpu, xt, d_in, d_out):
+    """NBPG_PCA_BF16 against an FP64 projection.  Each operand is rounded to
+    bf16 (relative error <= 2^-9), products are exact in fp32 and sums carry
+    at most d_in * 2^-24 relative error, so |y - y64| <= 2^-7 * sum|w||x|
+    holds with a 2x margin.  The fused codes equal the quantizer applied to
+    the kernel's own y."""
+    N = gpu
+    import torch
+    rng = np.random.default_rng(d_in + d_out)
+    n = 4099
+    if xt == "f32":
+        x = (rng.standard_normal((n, d_in)) * 0.05).astype(np.float32)
+        xf = x.astype(np.float64)
+    else:
+        x = rng.integers(0, 256, (n, d_in), dtype=np.uint8)
+        if xt == "i8":
+            x = x.view(np.int8)
+        xf = x.astype(np.float64)
+    W = (rng.standard_normal((d_out, d_in)) / (8.0 * np.sqrt(d_in) * (1 if xt == "f32" else 120))
+         ).astype(np.float32)
+    b = (rng.standard_normal(d_out) * 0.05).astype(np.float32)
+    xd, Wd, bd = (torch.from_numpy(a).cuda() for a in (x, W, b))
+    y64 = xf @ W.astype(np.float64).T + b
+    bound = np.abs(xf) @ np.abs(W.astype(np.float64)).T * 2.0 ** -7 + 1e-6
+    for code, sg, ch, _ in CODES:
+        c, y = N.pca_project(xd, Wd, bd, code=code, mode=N.PCA_BF16, want_y=True)
+        y = y.cpu().numpy()
+        err = np.abs(y - y64)
+        assert (err <= bound).all(), (code, err.max())
+        assert (c.cpu().numpy() == O.compress(y, sg, ch)).all(), code
+    print("bf16 %s %dx%d: max abs err %.3g, mean %.3g" % (xt, d_in, d_out, err.max(), err.mean()))
+
+
+def test_stats_f64(gpu):
+    """FP64 mean and covariance against numpy's float64, d not a multiple of
+    the 64-column tiles, f32 and u8 rows."""
+    N = gpu
+    import torch
+    rng = np.random.default_rng(5)
+    for x in [(rng.standard_normal((7001, 100)) * np.linspace(0.1, 3, 100)).astype(np.float32),
+              rng.integers(0, 256, (3000, 70), dtype=np.uint8)]:
+        mean, cov = N.pca_stats(torch.from_numpy(x).cuda())
+        xf = x.astype(np.float64)
+        m64 = xf.mean(axis=0)
+        c64 = np.cov(xf, rowvar=False)
+        assert np.allclose(mean.cpu().numpy(), m64, rtol=1e-12, atol=1e-12)
+        c = cov.cpu().numpy()
+        assert (c == c.T).all()
+        assert np.abs(c - c64).max() <= 1e-10 * np.abs(c64).max()
+
+
+def test_fit_recovers_subspace(gpu):
+    """Rows on a 3-dimensional subspace plus small noise: the fit's first
+    three directions span it, the projection of a sample lies in [-1, 1]."""
+    N = gpu
+    import torch
+    from nbpg_amd import pca as P
+    rng = np.random.default_rng(8)
+    Q, _ = np.linalg.qr(rng.standard_normal((40, 3)))
+    z = rng.standard_normal((20000, 3)) * np.array([5.0, 3.0, 2.0])
+    x = (z @ Q.T + 0.01 * rng.standard_normal((20000, 40)) + 7.0).astype(np.float32)
+    f = P.fit(torch.from_numpy(x).cuda(), 3)
+    V = f["W"].astype(np.float64) * f["scale"]
+    assert np.allclose(V @ V.T, np.eye(3), atol=1e-5)
+    assert np.linalg.norm(V @ Q @ Q.T - V) < 1e-3
+    assert f["eigenvalues"][0] > f["eigenvalues"][1] > f["eigenvalues"][2] > 100 * f["eigenvalues"][3]
+    _, y = N.pca_project(torch.from_numpy(x).cuda(), torch.from_numpy(f["W"]).cuda(),
+                         torch.from_numpy(f["bias"]).cuda(), code=N.CODE_F32, mode=N.PCA_PLAIN)
+    assert np.abs(y.cpu().numpy()).max() <= 1.0 + 1e-5

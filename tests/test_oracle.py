@@ -1,4 +1,6 @@
 """The CPU oracle against the reference's recorded outputs and closed forms."""
+import os
+
 import numpy as np
 import pytest
 
@@ -192,6 +194,30 @@ def test_recall_with_ties():
     # k = 2: gt[1] has dist 1, gt[2] ties it -> 3 admissible ids
     assert O.recall(np.array([[0, 2]], np.uint32), gt, base, q, O.U8, 2) == 1.0
     assert O.recall(np.array([[0, 3]], np.uint32), gt, base, q, O.U8, 2) == 0.5
+
+
+def test_pca_compress_reference_golden():
+    """matmul + compress == the reference's PCACompress binary, .fbin and all
+    four code types, with and without a bias in the .mat."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_pca.npz"))
+    for t in "fg":
+        y = O.matmul(g["x_" + t], g["W_" + t], g["b_" + t])
+        assert (y == g["out_%s_fbin" % t]).all()
+        for sg, ch, e in [(0, 1, "u8bin"), (1, 1, "i8bin"), (0, 2, "u4bin"), (1, 2, "i4bin")]:
+            assert (O.compress(y, sg, ch) == g["out_%s_%s" % (t, e)]).all(), (t, e)
+
+
+def test_mat_file_roundtrip(tmp_path):
+    from nbpg_amd import pca as P
+    W = np.arange(12, dtype=np.float32).reshape(3, 4)
+    P.save_mat(str(tmp_path / "a.mat"), W, np.array([1, 2, 3], np.float32))
+    W2, b2 = P.load_mat(str(tmp_path / "a.mat"))
+    assert (W2 == W).all() and b2.tolist() == [1, 2, 3]
+    raw = (tmp_path / "a.mat").read_bytes()
+    assert len(raw) == 9 + 4 * 12 + 4 * 3 and raw[:8] == np.array([3, 4], np.uint32).tobytes()
+    P.save_mat(str(tmp_path / "b.mat"), W)
+    W3, b3 = P.load_mat(str(tmp_path / "b.mat"))
+    assert (W3 == W).all() and (b3 == 0).all()
 
 
 def test_matmul_against_float64():
