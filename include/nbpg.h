@@ -192,6 +192,45 @@ int nbpg_pca_quantize(const float *d_y, uint64_t n, uint32_t d, int code, uint8_
 int nbpg_pca_stats_f64(int x_dtype, const void *d_x, uint64_t n, uint32_t d, double *d_mean,
                        double *d_cov, void *stream);
 
+/* ---- serving: a continuous-batching request loop over a resident index.
+ * No counterpart in the reference (its searches run as whole batches,
+ * check_nn_recall.h:74-76).  Requests of 1.. queries (host rows, own L and
+ * k) are queued from any thread; a scheduler thread keeps `slots` batches in
+ * flight on their own streams and admits queued requests, in arrival order,
+ * into each slot that frees up, up to the slot's query capacity:
+ * min(max_batch_queries, mem_budget / slots / bytes per query) where a query
+ * takes row_stride + 8 + 8 * max_k device bytes (mem_budget 0 = no limit).
+ * Each query runs the FAST search with its own L and k, and its result does
+ * not depend on the other queries of its batch. ---- */
+typedef struct {
+    uint32_t slots;             /* batches in flight, 1..8 */
+    uint32_t max_batch_queries; /* queries per batch */
+    uint64_t mem_budget;        /* device bytes for all slots' buffers (0 = unbounded) */
+    uint32_t max_L, max_k;      /* largest L and k a request may ask for */
+    uint32_t entry;             /* entry node */
+    nbpg_fast_config fast;      /* launch shape of every batch */
+} nbpg_server_config;
+
+typedef struct {
+    uint64_t requests, queries, batches; /* completed since creation / last reset */
+    double mean_batch_queries;
+    double p50_ms, p99_ms, max_ms; /* request latency: submit -> results on the host */
+    uint32_t batch_capacity;       /* queries per batch admitted */
+} nbpg_server_stats_t;
+
+typedef struct nbpg_server nbpg_server;
+/* the index must outlive the server */
+int nbpg_server_create(nbpg_server **out, const nbpg_index *idx, const nbpg_server_config *cfg);
+/* copies the rows; returns a ticket to wait on (once) */
+int nbpg_server_submit(nbpg_server *srv, const void *host_queries, uint32_t nq,
+                       uint64_t query_stride, uint32_t L, uint32_t k, uint64_t *ticket);
+/* blocks until the request is done; nq x k ids / dists (either may be NULL) */
+int nbpg_server_wait(nbpg_server *srv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
+                     double *latency_ms);
+int nbpg_server_stats(nbpg_server *srv, int reset, nbpg_server_stats_t *stats);
+/* finishes queued requests, then frees the slots */
+int nbpg_server_destroy(nbpg_server *srv);
+
 /* ---- graph build: build_vamana (nsw.h:547-663) on the device; the
  * index's points must be set; the graph is (re)built in place (set a graph
  * first to continue from it, Vamana.cpp:174-177). */

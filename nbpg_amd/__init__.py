@@ -54,6 +54,17 @@ class BuildParams(C.Structure):
 VP = C.c_void_p
 U32, U64, I32, DBL = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", U32), ("max_batch_queries", U32), ("mem_budget", U64),
+                ("max_L", U32), ("max_k", U32), ("entry", U32), ("fast", FastConfig)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", U64), ("queries", U64), ("batches", U64),
+                ("mean_batch_queries", DBL), ("p50_ms", DBL), ("p99_ms", DBL), ("max_ms", DBL),
+                ("batch_capacity", U32)]
+
 _SIGS = {
     "nbpg_last_error": (C.c_char_p, []),
     "nbpg_init": (I32, [I32]),
@@ -74,6 +85,11 @@ _SIGS = {
     "nbpg_robust_prune_batch": (I32, [VP, VP, U64, VP, VP, U32, VP, DBL, DBL, VP, VP, VP, VP]),
     "nbpg_compgt": (I32, [I32, VP, U64, VP, U64, U32, U32, I32, VP, VP, VP]),
     "nbpg_build": (I32, [VP, C.POINTER(BuildParams), C.POINTER(U64)]),
+    "nbpg_server_create": (I32, [C.POINTER(VP), VP, C.POINTER(ServerConfig)]),
+    "nbpg_server_submit": (I32, [VP, VP, U32, U64, U32, U32, C.POINTER(U64)]),
+    "nbpg_server_wait": (I32, [VP, U64, VP, VP, C.POINTER(DBL)]),
+    "nbpg_server_stats": (I32, [VP, I32, C.POINTER(ServerStats)]),
+    "nbpg_server_destroy": (I32, [VP]),
     "nbpg_pca_project": (I32, [I32, VP, U64, U32, VP, VP, U32, I32, I32, VP, VP, VP]),
     "nbpg_pca_quantize": (I32, [VP, U64, U32, I32, VP, VP]),
     "nbpg_pca_stats_f64": (I32, [I32, VP, U64, U32, VP, VP, VP]),
@@ -219,6 +235,56 @@ def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None):
                                     max_L, out_k, entry, C.byref(fc) if fc is not None else None,
                                     _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search_uneven")
     return ids, d, st
+
+
+class Server:
+    """Continuous-batching request loop over a resident index (see
+    include/nbpg.h).  submit() and wait() may be called from any thread;
+    wait() releases the GIL while it blocks."""
+
+    def __init__(self, index, slots=3, max_batch_queries=4096, mem_budget=0, max_L=256,
+                 max_k=100, entry=0, fast=(2, 1, 11, 0, 0)):
+        _torch()
+        self.index = index
+        cfg = ServerConfig(slots, max_batch_queries, mem_budget, max_L, max_k, entry,
+                           FastConfig(*fast))
+        h = VP()
+        _check(lib().nbpg_server_create(C.byref(h), index.h, C.byref(cfg)), "nbpg_server_create")
+        self.h = h
+        self._shape = {}
+
+    def submit(self, rows, L, k):
+        a = np.ascontiguousarray(rows, dtype=np.uint8)
+        t = U64()
+        _check(lib().nbpg_server_submit(self.h, a.ctypes.data, a.shape[0], a.shape[1], L, k,
+                                        C.byref(t)), "nbpg_server_submit")
+        self._shape[t.value] = (a.shape[0], k)
+        return t.value
+
+    def wait(self, ticket):
+        nq, k = self._shape.pop(ticket)
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        lat = DBL()
+        _check(lib().nbpg_server_wait(self.h, ticket, ids.ctypes.data, d.ctypes.data,
+                                      C.byref(lat)), "nbpg_server_wait")
+        return ids, d, lat.value
+
+    def stats(self, reset=False):
+        st = ServerStats()
+        _check(lib().nbpg_server_stats(self.h, int(reset), C.byref(st)), "nbpg_server_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nbpg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def robust_prune_batch(index, nodes_dev, cand_dev, offsets_dev, R, alpha=1.2, cos_angle=0.0,

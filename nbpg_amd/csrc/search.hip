@@ -321,11 +321,13 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     return NBPG_OK;
 }
 
-static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
-                       uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L,
-                       const uint32_t *d_k, uint32_t Lmax, uint32_t out_k, uint64_t limit,
-                       const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
-                       uint32_t *d_stats, hipStream_t stream)
+// counter: the launch's work counter (a u32 in device memory); launches
+// that may run at the same time need their own
+int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
+                uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
+                uint32_t Lmax, uint32_t out_k, uint64_t limit, const nbpg_fast_config *cfg,
+                uint32_t *d_ids, float *d_dists, uint32_t *d_stats, uint32_t *counter,
+                hipStream_t stream)
 {
     nbpg_fast_config c = {2, 1, 11, 0, 1};
     if (cfg)
@@ -369,7 +371,7 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     // register beam: filter (+ the row-split kernel's compact list of 64*ipl ids)
     a.wave_bytes = reg ? (4u << c.filter_log2) + (split ? 256u * ipl : 0u)
                        : fast_wave_bytes(Lmax, ipl, c.filter_log2);
-    a.counter = ix->counter;
+    a.counter = counter;
     a.out_ids = d_ids;
     a.out_dists = d_dists;
     a.out_stats = d_stats;
@@ -387,7 +389,7 @@ static int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     uint64_t need = (nq + c.waves_per_group - 1) / c.waves_per_group;
     if (groups > need)
         groups = (uint32_t)std::max<uint64_t>(1, need);
-    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, ix->counter);
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     dim3 g(groups), b(64 * c.waves_per_group);
     if (reg)
         return launch_search_reg(ix->dtype, ix->nch, ipl, E, split, g, b, (uint32_t)lds, stream, a);
@@ -441,7 +443,7 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
             return set_error(NBPG_EUNSUP, "nq too large");
         return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], qp->L, qp->k, nullptr,
                            nullptr, qp->L, qp->k, qp->limit, fast, d_out_ids, d_out_dists,
-                           d_out_stats, as_stream(stream));
+                           d_out_stats, ix->counter, as_stream(stream));
     }
     return set_error(NBPG_EINVAL, "search mode %d", mode);
 }
@@ -464,7 +466,8 @@ int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     if (nq >= 0xFFFFFFFFull)
         return set_error(NBPG_EUNSUP, "nq too large");
     return search_fast(ix, d_queries, nq, query_stride, entry, 0, 0, d_L, d_k, max_L, out_k,
-                       ix->n, fast, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
+                       ix->n, fast, d_out_ids, d_out_dists, d_out_stats, ix->counter,
+                       as_stream(stream));
 }
 
 } // extern "C"

@@ -40,6 +40,13 @@ __device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
     return (id * 0x9E3779B1u) >> (32 - flog2);
 }
 
+// FAST-mode launch (search.hip); counter is the launch's own work counter
+int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
+                uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
+                uint32_t Lmax, uint32_t out_k, uint64_t limit, const nbpg_fast_config *cfg,
+                uint32_t *d_ids, float *d_dists, uint32_t *d_stats, uint32_t *counter,
+                hipStream_t stream);
+
 int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
                       dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a);
 

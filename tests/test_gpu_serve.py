@@ -1,0 +1,118 @@
+"""Serving loop: each request's results are the same alone and in a mix of
+concurrent requests (bit-identical ids and distances), and equal to a direct
+uneven-batch search of the same queries."""
+import threading
+
+import numpy as np
+import pytest
+
+from nbpg_amd import datagen as D
+
+pytestmark = pytest.mark.gpu
+
+FAST = (2, 1, 11, 0, 0)
+
+
+@pytest.fixture(scope="module")
+def served(gpu):
+    N = gpu
+    base, q = D.workload_small(30000, 600, dims=64)
+    ix = N.Index(N.U8, base.shape[0], base.shape[1], 32)
+    ix.set_points(base)
+    N.build(ix, 32, 64, alpha=1.2, passes=1, num_split=0)
+    rng = np.random.default_rng(9)
+    reqs, off = [], 0
+    while off < len(q):
+        nq = int(min(rng.integers(1, 65), len(q) - off))
+        L = int(np.exp(rng.uniform(np.log(16), np.log(256))))
+        k = int(rng.choice([1, 10, 100]))
+        k = min(k, L)
+        reqs.append((q[off:off + nq], L, k))
+        off += nq
+    yield N, ix, reqs
+    ix.close()
+
+
+def run_alone(N, ix, reqs, **kw):
+    srv = N.Server(ix, fast=FAST, **kw)
+    out = []
+    for rows, L, k in reqs:
+        t = srv.submit(rows, L, k)
+        ids, d, lat = srv.wait(t)
+        assert lat > 0
+        out.append((ids, d))
+    srv.close()
+    return out
+
+
+def test_same_alone_and_in_mix(served):
+    N, ix, reqs = served
+    alone = run_alone(N, ix, reqs)
+    srv = N.Server(ix, slots=3, max_batch_queries=160, fast=FAST)
+    order = np.random.default_rng(1).permutation(len(reqs))
+    tickets = {}
+    lock = threading.Lock()
+
+    def producer(part):
+        for i in part:
+            rows, L, k = reqs[i]
+            t = srv.submit(rows, L, k)
+            with lock:
+                tickets[i] = t
+
+    threads = [threading.Thread(target=producer, args=(order[j::4],)) for j in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    for i in range(len(reqs)):
+        ids, d, _ = srv.wait(tickets[i])
+        assert (ids == alone[i][0]).all(), i
+        assert (d == alone[i][1]).all(), i
+    st = srv.stats()
+    assert st["requests"] == len(reqs)
+    assert st["queries"] == sum(len(r[0]) for r in reqs)
+    assert st["batches"] > 1 and st["mean_batch_queries"] > 20
+    assert st["batch_capacity"] == 160
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    srv.close()
+
+
+def test_matches_direct_uneven_search(served):
+    N, ix, reqs = served
+    import torch
+    alone = run_alone(N, ix, reqs[:12])
+    rows = np.concatenate([r[0] for r in reqs[:12]])
+    Ls = np.concatenate([[r[1]] * len(r[0]) for r in reqs[:12]]).astype(np.int32)
+    ks = np.concatenate([[r[2]] * len(r[0]) for r in reqs[:12]]).astype(np.int32)
+    stride = ix.row_stride
+    qpad = np.zeros((len(rows), stride), np.uint8)
+    qpad[:, :rows.shape[1]] = rows
+    ids, d, _ = N.search_uneven(ix, N.to_device(qpad), torch.from_numpy(Ls).cuda(),
+                                torch.from_numpy(ks).cuda(), 256, 100, fast=FAST)
+    ids = ids.cpu().numpy().view(np.uint32)
+    d = d.cpu().numpy()
+    off = 0
+    for (r, L, k), (ai, ad) in zip(reqs[:12], alone):
+        assert (ids[off:off + len(r), :k] == ai).all()
+        assert (d[off:off + len(r), :k] == ad).all()
+        off += len(r)
+
+
+def test_memory_admission_and_errors(served):
+    N, ix, reqs = served
+    stride = ix.row_stride
+    per_query = stride + 8 + 8 * 100
+    srv = N.Server(ix, slots=2, max_batch_queries=4096, mem_budget=2 * 50 * per_query, fast=FAST)
+    assert srv.stats()["batch_capacity"] == 50
+    with pytest.raises(N.NbpgError):
+        srv.submit(np.zeros((51, 64), np.uint8), 32, 10)  # larger than a batch
+    with pytest.raises(N.NbpgError):
+        srv.submit(np.zeros((4, 64), np.uint8), 16, 32)  # k > L
+    with pytest.raises(N.NbpgError):
+        srv.submit(np.zeros((4, 64), np.uint8), 512, 10)  # L > max_L
+    rows, L, k = reqs[0]
+    t = srv.submit(rows, L, k)
+    ids, _, _ = srv.wait(t)
+    assert (ids < ix.n).all()
+    srv.close()
