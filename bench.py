@@ -40,20 +40,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--n", type=int, default=0, help="base rows (0 = the workload's own size)")
+    ap.add_argument("--nq", type=int, default=0, help="queries (0 = the workload's own count)")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--L", type=int, default=0, help="fixed beam width (0 = sweep)")
     ap.add_argument("--fast", default="auto",
                     help="waves_per_group,expand_width,filter_log2,groups,row_split or 'auto'")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a hipGraph")
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device (HBM counter calibration)")
     ap.add_argument("--calibrate-gather", action="store_true",
                     help="only measure random 128-byte row gathers (index_select)")
+    ap.add_argument("--workload", default="search",
+                    choices=["search", "c1", "pca", "uneven", "serve"],
+                    help="search: the headline 1M x 128 u8 batched search (default); c1: 100k x "
+                         "128 u8 reference-exact build + search; pca: 1M x 960 f32 PCACompress + "
+                         "search on codes; uneven: uneven batches + batched RobustPrune; serve: "
+                         "10M x 96 i8 continuous-batching server under Poisson load")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -77,6 +83,11 @@ def main():
     if args.calibrate_gather:
         print(json.dumps(gather_ceiling(torch)), flush=True)
         return
+    if args.workload != "search":
+        import workloads  # noqa: E402  (bench-only module beside this file)
+        res = getattr(workloads, "run_" + args.workload)(args, log)
+        print(json.dumps(res), flush=True)
+        return
     import nbpg_amd as N
     from nbpg_amd import autotune as AT
     from nbpg_amd import datagen as D
@@ -84,7 +95,7 @@ def main():
     N.init(0)
     dev = N.device_info()
     log("device", dev)
-    base, q = D.workload_1m(args.n, args.nq)
+    base, q = D.workload_1m(args.n or 1_000_000, args.nq or 10_000)
     log("data", base.shape, q.shape, "zeros %.3f" % float((base == 0).mean()))
     R = 64
     ix = N.Index(N.U8, base.shape[0], base.shape[1], R)

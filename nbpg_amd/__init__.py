@@ -254,7 +254,8 @@ class Server:
         self._shape = {}
 
     def submit(self, rows, L, k):
-        a = np.ascontiguousarray(rows, dtype=np.uint8)
+        a = np.ascontiguousarray(rows)
+        a = a.view(np.uint8).reshape(a.shape[0], -1)
         t = U64()
         _check(lib().nbpg_server_submit(self.h, a.ctypes.data, a.shape[0], a.shape[1], L, k,
                                         C.byref(t)), "nbpg_server_submit")

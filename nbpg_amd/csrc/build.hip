@@ -23,6 +23,9 @@
 #include <math.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <vector>
 
@@ -408,6 +411,19 @@ static int download_graph(nbpg_index *ix, std::vector<uint32_t> &nbr, std::vecto
     return NBPG_OK;
 }
 
+// progress lines on stderr when NBPG_BUILD_LOG is set (long builds); the
+// batch loop then waits for each batch so the lines track the device
+static bool build_log_on()
+{
+    static const bool on = getenv("NBPG_BUILD_LOG") != nullptr;
+    return on;
+}
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
 static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int pop_mode, int sp_mode)
 {
     nbpg_index *ix = c.ix;
@@ -487,6 +503,8 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     ra.key2 = c.keyA;
 
     uint64_t inc = 0, count = 0;
+    const bool logging = build_log_on();
+    double t_last = logging ? now_s() : 0.0;
     while (count < m) {
         // next_batch_range, base 2 (nsw.h:30-47)
         uint64_t fl, ce;
@@ -547,6 +565,12 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
         });
         NBPG_TRY(rc);
         HIP_TRY(hipGetLastError());
+        if (logging && now_s() - t_last > 5.0) {
+            HIP_TRY(hipStreamSynchronize(c.s));
+            fprintf(stderr, "[nbpg build] inserted %llu / %llu\n", (unsigned long long)count,
+                    (unsigned long long)m);
+            t_last = now_s();
+        }
     }
     HIP_TRY(hipStreamSynchronize(c.s));
 
@@ -702,6 +726,8 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
         for (int i = 0; i < 2 && rc == NBPG_OK; ++i) {
             std::shuffle(ins.begin(), ins.end(), std::mt19937(i));
             rc = batch_insert(c, ins, pop, SP_SPLIT);
+            if (build_log_on())
+                fprintf(stderr, "[nbpg build] split pass %d done\n", i);
         }
     for (uint32_t i = 0; i < bp->num_passes && rc == NBPG_OK; ++i) { // nsw.h:618-654
         std::shuffle(ins.begin(), ins.end(), std::mt19937(i));
@@ -715,6 +741,8 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
         else
             spm = bp->rand_sp ? SP_RAND : SP_DEFAULT;
         rc = batch_insert(c, ins, pop, spm);
+        if (build_log_on())
+            fprintf(stderr, "[nbpg build] pass %u done\n", i);
     }
     uint64_t iso = 0;
     if (rc == NBPG_OK && bp->repair_isolate) {

@@ -60,6 +60,23 @@ int ensure_ws(nbpg_index *idx, size_t bytes)
     return NBPG_OK;
 }
 
+// dst row r (stride bytes, zero padded) = the first `bytes` of src row r
+__global__ void spread_rows_kernel(const uint8_t *__restrict__ src, uint64_t src_stride, uint64_t m,
+                                   uint32_t bytes, uint8_t *__restrict__ dst, uint32_t stride)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wpr = stride / 4;
+    if (t >= m * wpr)
+        return;
+    const uint64_t r = t / wpr;
+    const uint32_t b0 = (uint32_t)(t % wpr) * 4;
+    uint32_t v = 0;
+    for (uint32_t j = 0; j < 4; ++j)
+        if (b0 + j < bytes)
+            v |= (uint32_t)src[r * src_stride + b0 + j] << (8 * j);
+    reinterpret_cast<uint32_t *>(dst + r * stride)[b0 / 4] = v;
+}
+
 // |row|^2 per row in the type's dot units
 template <int DT, int NCH>
 __global__ void norms_kernel(const uint8_t *__restrict__ vecs, uint32_t stride, uint64_t first,
@@ -315,8 +332,36 @@ int nbpg_index_set_points(nbpg_index *ix, const void *host_rows, uint64_t host_s
                          (unsigned long long)ix->n, (unsigned long long)host_stride);
     if (count == 0)
         return NBPG_OK;
-    HIP_TRY(hipMemcpy2D(ix->vecs + first * ix->stride, ix->stride, host_rows, host_stride, ix->dims,
-                        count, hipMemcpyHostToDevice));
+    if (host_stride == ix->stride) {
+        HIP_TRY(hipMemcpy(ix->vecs + first * ix->stride, host_rows, count * ix->stride,
+                          hipMemcpyHostToDevice));
+    } else {
+        // contiguous host chunks into a device staging buffer, then a kernel
+        // spreads the rows to the padded stride (one strided copy of many
+        // short rows is far slower than a flat copy)
+        const uint64_t rows_per = std::max<uint64_t>(1, (64ull << 20) / host_stride);
+        const uint64_t chunk = std::min<uint64_t>(rows_per, count);
+        uint8_t *stage = nullptr;
+        if (hipMalloc(&stage, chunk * host_stride) != hipSuccess)
+            return set_error(NBPG_ENOMEM, "point staging");
+        for (uint64_t s0 = 0; s0 < count; s0 += chunk) {
+            const uint64_t m = std::min<uint64_t>(chunk, count - s0);
+            const hipError_t e = hipMemcpy(stage, (const uint8_t *)host_rows + s0 * host_stride,
+                                           m * host_stride, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                hipFree(stage);
+                return set_error(NBPG_EHIP, "point upload: %s", hipGetErrorString(e));
+            }
+            const uint64_t words = m * (ix->stride / 4);
+            hipLaunchKernelGGL(spread_rows_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256),
+                               0, 0, stage, host_stride, m, ix->dims, ix->vecs + (first + s0) * ix->stride,
+                               ix->stride);
+        }
+        const hipError_t e = hipDeviceSynchronize();
+        hipFree(stage);
+        if (e != hipSuccess)
+            return set_error(NBPG_EHIP, "point spread: %s", hipGetErrorString(e));
+    }
     int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
         uint32_t blocks = (uint32_t)((count + 255) / 256);
         hipLaunchKernelGGL((norms_kernel<DT.value, NCH.value>), dim3(blocks), dim3(256), 0, 0,

@@ -71,3 +71,51 @@ def workload_1m(n=1_000_000, nq=10_000, dims=128):
     base = mixture_u8(n, dims, 1000, 25.0, seed=3, centers_seed=9, scale=sc, shift=66.0)
     queries = mixture_u8(nq, dims, 1000, 25.0, seed=4, centers_seed=9, scale=sc, shift=66.0)
     return base, queries
+
+
+def workload_serve(n=10_000_000, nq=10_000, dims=96, seed=21):
+    """10M x 96 i8 rows in -127..127: 4096 clusters (centres U[-90, 90]),
+    sigma 30.  Drawn on the device (torch's seeded generator) and returned
+    as host arrays; base and queries share the centres."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    centers = torch.rand((4096, dims), generator=g, device="cuda") * 180.0 - 90.0
+
+    def draw(m):
+        out = torch.empty((m, dims), dtype=torch.int8, device="cuda")
+        for s in range(0, m, 1 << 21):
+            e = min(m, s + (1 << 21))
+            lab = torch.randint(0, 4096, (e - s,), generator=g, device="cuda")
+            x = centers[lab] + torch.randn((e - s, dims), generator=g, device="cuda") * 30.0
+            out[s:e] = torch.clamp(torch.round(x), -127, 127).to(torch.int8)
+        return out
+
+    base = draw(n)
+    queries = draw(nq)
+    return base.cpu().numpy(), queries.cpu().numpy()
+
+
+def workload_pca(n=1_000_000, nq=10_000, dims=960, clusters=1000, seed=31):
+    """f32 rows from a Gaussian mixture with a decaying spectrum: in a random
+    orthonormal basis Q, per-coordinate variance i^-1.2 (i = 1..dims), with
+    cluster centres drawn from the same spectrum (scale 2).  Returned on the
+    device: (base n x dims, queries nq x dims)."""
+    import torch
+    rng = _rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((dims, dims)))
+    sd = (np.arange(1, dims + 1, dtype=np.float64) ** -0.6)
+    B = torch.from_numpy((Q * sd[None, :]).T.astype(np.float32)).cuda()  # z @ B = (z*sd) Q^T
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    centers = 2.0 * (torch.randn((clusters, dims), generator=g, device="cuda") @ B)
+
+    def draw(m):
+        out = torch.empty((m, dims), dtype=torch.float32, device="cuda")
+        for s in range(0, m, 1 << 18):
+            e = min(m, s + (1 << 18))
+            lab = torch.randint(0, clusters, (e - s,), generator=g, device="cuda")
+            out[s:e] = centers[lab] + torch.randn((e - s, dims), generator=g, device="cuda") @ B
+        return out
+
+    return draw(n), draw(nq)

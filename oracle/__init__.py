@@ -125,6 +125,10 @@ def ref():
         R.nbpgr_prune.restype = C.c_uint64
         R.nbpgr_prune.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                   C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_double]
+        R.nbpgr_prune_batch.restype = C.c_double
+        R.nbpgr_prune_batch.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                                        C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint32,
+                                        C.c_double, C.c_double, C.c_void_p]
         R.nbpgr_shuffle.restype = None
         R.nbpgr_shuffle.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32]
         R.nbpgr_matmul.restype = None

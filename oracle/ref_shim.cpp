@@ -93,6 +93,31 @@ uint64_t prune_t(const uint8_t *base, uint64_t n, uint32_t dims, uint32_t p, IdD
     return sz;
 }
 
+// many nodes over one PointRange; OpenMP over nodes like the reference's
+// build loops; returns the seconds of the prune loop alone
+template <typename T>
+double prune_batch_t(const uint8_t *base, uint64_t n, uint32_t dims, const uint32_t *nodes,
+                     uint64_t P, IdDist<uint32_t> *cand, uint32_t C, uint32_t R, double alpha,
+                     double cos_angle, uint32_t *out_counts)
+{
+    auto B = make_points<T>(base, n, dims);
+    Graph<uint32_t> G(1, 1, false);
+    const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel
+    {
+        Prune pr(R, 1);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t t = 0; t < (int64_t)P; ++t) {
+            size_t sz = C;
+            IdDist<uint32_t> *c = cand + (size_t)t * C;
+            pr.sort_candidates(nodes[t], G, c, sz, 0.0f);
+            pr.robustPrune(nodes[t], c, sz, B, alpha, cos_angle, 1);
+            out_counts[t] = (uint32_t)sz;
+        }
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 } // namespace
 
 extern "C" {
@@ -141,6 +166,20 @@ uint64_t nbpgr_prune(int dtype, const uint8_t *base, uint64_t n, uint32_t dims, 
     case 3: return prune_t<int4_2>(base, n, dims, p, c, size, R, alpha, cos_angle);
     }
     return 0;
+}
+
+double nbpgr_prune_batch(int dtype, const uint8_t *base, uint64_t n, uint32_t dims,
+                         const uint32_t *nodes, uint64_t P, void *cand, uint32_t C, uint32_t R,
+                         double alpha, double cos_angle, uint32_t *out_counts)
+{
+    auto *c = reinterpret_cast<IdDist<uint32_t> *>(cand);
+    switch (dtype) {
+    case 0: return prune_batch_t<uint8_t>(base, n, dims, nodes, P, c, C, R, alpha, cos_angle, out_counts);
+    case 1: return prune_batch_t<int8_t>(base, n, dims, nodes, P, c, C, R, alpha, cos_angle, out_counts);
+    case 2: return prune_batch_t<uint4_2>(base, n, dims, nodes, P, c, C, R, alpha, cos_angle, out_counts);
+    case 3: return prune_batch_t<int4_2>(base, n, dims, nodes, P, c, C, R, alpha, cos_angle, out_counts);
+    }
+    return -1.0;
 }
 
 void nbpgr_shuffle(uint32_t *a, uint64_t n, uint32_t seed)

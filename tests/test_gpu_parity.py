@@ -50,6 +50,27 @@ def test_l2_batch_exact(gpu):
         ix.close()
 
 
+def test_set_points_strided_and_partial(gpu):
+    """Rows uploaded from a wider host stride, in two ranges, land padded
+    (the staging + spread path) and give exact distances."""
+    N = gpu
+    rng = np.random.default_rng(4)
+    for dims, hstride in [(20, 24), (100, 100), (96, 112)]:
+        wide = rng.integers(0, 256, (3000, hstride), dtype=np.uint8)
+        base = wide[:, :dims]
+        ix = N.Index(N.U8, 3000, dims, 8)
+        for a, b in [(1234, 3000), (0, 1234)]:
+            part = np.ascontiguousarray(wide[a:b])
+            N._check(N.lib().nbpg_index_set_points(ix.h, part.ctypes.data, hstride, a, b - a),
+                     "set_points")
+        q = rng.integers(0, 256, dims, dtype=np.uint8)
+        ids = np.arange(3000, dtype=np.uint32)
+        out = N.l2_batch(ix, N.to_device(q), N.to_device(ids)).cpu().numpy()
+        ref = np.array([O.distance(O.U8, base[i], q) for i in ids], np.float32)
+        assert (out == ref).all(), dims
+        ix.close()
+
+
 def test_search_ref_matches_reference_golden(gpu, golden):
     N = gpu
     base, q, g1 = golden["base"], golden["queries"], golden["graph1"]
