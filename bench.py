@@ -48,6 +48,8 @@ def main():
                     help="waves_per_group,expand_width,filter_log2,groups,row_split or 'auto'")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a hipGraph")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--seeds", type=int, default=64,
+                    help="FAST-mode start nodes from k-means over the base (1 = node 0 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -147,10 +149,18 @@ def main():
     def recall_at(ids, g):
         return AT.recall_with_ties(ids, g, gtd[:len(g)], 10)
 
+    # FAST-mode start nodes: 64 k-means seeds over the base (the plain path
+    # keeps the reference's single entry, node 0)
+    seeds = None
+    if args.seeds > 1:
+        t = time.time()
+        seeds = N.pick_seeds(base_d, args.seeds)
+        log("%d k-means seeds in %.2fs" % (len(seeds), time.time() - t))
     tune_table = None
     if args.fast == "auto":
         t = time.time()
-        best, tune_table = AT.tune(ix, q_d, gt, gtd, k=10, target=args.target, log=log)
+        best, tune_table = AT.tune(ix, q_d, gt, gtd, k=10, target=args.target, log=log,
+                                   start_points=seeds)
         fast = best["config"]
         log("autotuner picked %s at L=%d in %.1fs" % (fast, best["L"], time.time() - t))
         if not args.L:
@@ -161,6 +171,10 @@ def main():
                                         200, 256]
 
     def run_fast(L, out=None):
+        return N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, out=out,
+                        start_points=seeds)
+
+    def run_fast_entry0(L, out=None):
         return N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, out=out)
 
     def run_plain(L, out=None):
@@ -179,6 +193,8 @@ def main():
     L_fast, rec_fast, st_fast = pick(run_fast, "fast")
     sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
     L_plain, rec_plain, st_plain = pick(run_plain, "plain")
+    L_e0, rec_e0, _ = pick(run_fast_entry0, "fast, entry node 0") if seeds is not None \
+        else (L_fast, rec_fast, None)
 
     def timed(runner, L, steps, warmup, graph):
         out = runner(L)
@@ -215,6 +231,11 @@ def main():
     qps = nq * args.steps / wall
     log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
         (L_fast, qps, np.percentile(per, 50), np.percentile(per, 99)))
+    e0_qps = qps
+    if seeds is not None:
+        w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2, not args.no_graph)
+        e0_qps = nq * max(5, args.steps // 2) / w0
+        log("fast with the single entry node 0, L=%d: %.0f QPS" % (L_e0, e0_qps))
     pwall, pper = timed(run_plain, L_plain, args.plain_steps, 1, False)
     plain_qps = nq * args.plain_steps / pwall
     log("plain L=%d: %.0f QPS" % (L_plain, plain_qps))
@@ -249,6 +270,8 @@ def main():
         "config": {"workload": "1M x 128 u8 base, 10k queries, NBPG graph R=64 built on device, "
                                "k=10, fast persistent search", "n": base.shape[0], "nq": nq,
                    "L": L_fast, "fast_config": list(fast), "recall@10": round(rec_fast, 4),
+                   "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
+                   else "node 0",
                    "hipgraph": not args.no_graph,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -258,6 +281,7 @@ def main():
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
                   "speedup_fast_over_plain": round(qps / plain_qps, 2)},
+        "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)},
         "build_s": round(build_s, 2),
         "groundtruth_s": round(gt_s, 3),
         "groundtruth_valu_s": round(gt_plain_s, 3),

@@ -57,7 +57,7 @@ def _time(fn, reps):
 
 
 def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAULT_L, reps=5,
-         log=None):
+         log=None, start_points=None):
     """Returns (best, table): best = dict(config, L, ms, qps, recall)."""
     import torch
     space = space or DEFAULT_SPACE
@@ -74,7 +74,8 @@ def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAU
             for L in Ls:
                 if L < k:
                     continue
-                ids, _, _ = N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg, stats=False)
+                ids, _, _ = N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg, stats=False,
+                                     start_points=start_points)
                 torch.cuda.synchronize()
                 r = recall_with_ties(ids.cpu().numpy().view(np.uint32), gt_ids, gt_dists, k)
                 if r >= target:
@@ -83,9 +84,10 @@ def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAU
             if chosen is None:
                 continue
             L, r = chosen
-            out = N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg, stats=False)
+            out = N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg, stats=False,
+                           start_points=start_points)
             ms = _time(lambda: N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg,
-                                        out=out), reps)
+                                        out=out, start_points=start_points), reps)
         except N.NbpgError as e:  # shape not supported (LDS budget, ...)
             if log:
                 log("autotune skip %s: %s" % (cfg, e))

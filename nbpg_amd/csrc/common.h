@@ -47,6 +47,21 @@ template <> struct dot_traits<NBPG_I4X2> : dot_traits<NBPG_U4X2> {};
 
 // the query row as wave-uniform words: every lane holds the same query, so
 // it lives in scalar registers and feeds the dot instructions' scalar operand
+// the query row as plain words in every lane's vector registers: for long
+// kernels whose scalar registers are already full (a spilled scalar operand
+// costs a v_readlane per use)
+template <int NCH>
+__device__ __forceinline__ void to_lanes(const uint4 (&qv)[NCH], uint32_t (&qs)[4 * NCH])
+{
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        qs[4 * c + 0] = qv[c].x;
+        qs[4 * c + 1] = qv[c].y;
+        qs[4 * c + 2] = qv[c].z;
+        qs[4 * c + 3] = qv[c].w;
+    }
+}
+
 template <int NCH>
 __device__ __forceinline__ void to_uniform(const uint4 (&qv)[NCH], uint32_t (&qs)[4 * NCH])
 {
@@ -104,6 +119,31 @@ __device__ __forceinline__ float dist_row(const uint4 *__restrict__ row, const u
         xq = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], xq);
         xq = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], xq);
         xq = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], xq);
+        xx = dot_traits<DT>::dot(x[c].x, x[c].x, xx);
+        xx = dot_traits<DT>::dot(x[c].y, x[c].y, xx);
+        xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
+        xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
+    }
+    return dist_from<DT>(xx, nq, xq);
+}
+
+// same, with the query read from LDS one 16-byte chunk at a time (every lane
+// reads the same address: a broadcast), so the query costs no registers
+template <int DT, int NCH>
+__device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, const uint4 *q, uint32_t nq)
+{
+    uint4 x[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        x[c] = row[c];
+    uint32_t xq = 0, xx = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const uint4 qc = q[c];
+        xq = dot_traits<DT>::dot(x[c].x, qc.x, xq);
+        xq = dot_traits<DT>::dot(x[c].y, qc.y, xq);
+        xq = dot_traits<DT>::dot(x[c].z, qc.z, xq);
+        xq = dot_traits<DT>::dot(x[c].w, qc.w, xq);
         xx = dot_traits<DT>::dot(x[c].x, x[c].x, xx);
         xx = dot_traits<DT>::dot(x[c].y, x[c].y, xx);
         xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);

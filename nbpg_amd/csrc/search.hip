@@ -324,12 +324,12 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
 // counter: the launch's work counter (a u32 in device memory); launches
 // that may run at the same time need their own
 int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
-                uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
-                uint32_t Lmax, uint32_t out_k, uint64_t limit, const nbpg_fast_config *cfg,
-                uint32_t *d_ids, float *d_dists, uint32_t *d_stats, uint32_t *counter,
-                hipStream_t stream)
+                uint32_t entry, const uint32_t *seeds, uint32_t nseeds, uint32_t L, uint32_t k,
+                const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
+                uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
+                uint32_t *d_stats, uint32_t *counter, hipStream_t stream)
 {
-    nbpg_fast_config c = {2, 1, 11, 0, 1};
+    nbpg_fast_config c = {2, 1, 11, 0, 0};
     if (cfg)
         c = *cfg;
     if (c.waves_per_group < 1 || c.waves_per_group > 16 || c.expand_width < 1 ||
@@ -361,6 +361,19 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     a.flog2 = c.filter_log2;
     a.entry = entry;
     a.limit = limit;
+    if (nseeds > 1) {
+        if (nseeds > 64)
+            return set_error(NBPG_EINVAL, "%u seeds (at most 64)", nseeds);
+        for (uint32_t i = 0; i < nseeds; ++i) {
+            if (seeds[i] >= ix->n)
+                return set_error(NBPG_EINVAL, "seed %u >= n", seeds[i]);
+            for (uint32_t j = 0; j < i; ++j)
+                if (seeds[j] == seeds[i])
+                    return set_error(NBPG_EINVAL, "seed %u repeated", seeds[i]);
+            a.sp[i] = seeds[i];
+        }
+        a.nseeds = nseeds;
+    }
     // beams of up to 256 entries live in registers (search_reg.hip); larger
     // beams use the LDS-resident kernel below
     const bool reg = Lmax <= 4 * WAVE;
@@ -369,7 +382,8 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
         return set_error(NBPG_EINVAL, "query rows must be 4-byte aligned");
     const bool split = c.row_split != 0;
     // register beam: filter (+ the row-split kernel's compact list of 64*ipl ids)
-    a.wave_bytes = reg ? (4u << c.filter_log2) + (split ? 256u * ipl : 0u)
+    a.wave_bytes = reg ? (split ? (4u << c.filter_log2) + 256u * ipl
+                                : (4u << c.filter_log2) + 16u * ix->nch)
                        : fast_wave_bytes(Lmax, ipl, c.filter_log2);
     a.counter = counter;
     a.out_ids = d_ids;
@@ -391,6 +405,8 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
         groups = (uint32_t)std::max<uint64_t>(1, need);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     dim3 g(groups), b(64 * c.waves_per_group);
+    if (nseeds > 1 && (!reg || split))
+        return set_error(NBPG_EUNSUP, "seeded FAST search needs L <= 256 and row_split = 0");
     if (reg)
         return launch_search_reg(ix->dtype, ix->nch, ipl, E, split, g, b, (uint32_t)lds, stream, a);
     int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
@@ -426,8 +442,11 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     if (query_stride < ix->dims)
         return set_error(NBPG_EINVAL, "query stride %llu < row bytes %u",
                          (unsigned long long)query_stride, ix->dims);
-    if (sp_num == 0 || !h_start_points || sp_num > 32 || sp_num > qp->L) // query_params.h:206-210
-        return set_error(NBPG_EINVAL, "start points: %u (1..min(32, L))", sp_num);
+    // query_params.h:206-210 for the reference mode; FAST takes up to 64 seeds
+    if (sp_num == 0 || !h_start_points ||
+        (mode == NBPG_SEARCH_REF && (sp_num > 32 || sp_num > qp->L)) || sp_num > 64)
+        return set_error(NBPG_EINVAL, "start points: %u (reference mode 1..min(32, L), fast 1..64)",
+                         sp_num);
     for (uint32_t s = 0; s < sp_num; ++s)
         if (h_start_points[s] >= ix->n)
             return set_error(NBPG_EINVAL, "start point %u >= n", h_start_points[s]);
@@ -437,13 +456,11 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, d_out_ids,
                           d_out_dists, d_out_stats, as_stream(stream));
     if (mode == NBPG_SEARCH_FAST) {
-        if (sp_num != 1)
-            return set_error(NBPG_EUNSUP, "fast search takes one entry point");
         if (nq >= 0xFFFFFFFFull)
             return set_error(NBPG_EUNSUP, "nq too large");
-        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], qp->L, qp->k, nullptr,
-                           nullptr, qp->L, qp->k, qp->limit, fast, d_out_ids, d_out_dists,
-                           d_out_stats, ix->counter, as_stream(stream));
+        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
+                           sp_num, qp->L, qp->k, nullptr, nullptr, qp->L, qp->k, qp->limit, fast,
+                           d_out_ids, d_out_dists, d_out_stats, ix->counter, as_stream(stream));
     }
     return set_error(NBPG_EINVAL, "search mode %d", mode);
 }
@@ -465,8 +482,8 @@ int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
         return NBPG_OK;
     if (nq >= 0xFFFFFFFFull)
         return set_error(NBPG_EUNSUP, "nq too large");
-    return search_fast(ix, d_queries, nq, query_stride, entry, 0, 0, d_L, d_k, max_L, out_k,
-                       ix->n, fast, d_out_ids, d_out_dists, d_out_stats, ix->counter,
+    return search_fast(ix, d_queries, nq, query_stride, entry, nullptr, 0, 0, 0, d_L, d_k, max_L,
+                       out_k, ix->n, fast, d_out_ids, d_out_dists, d_out_stats, ix->counter,
                        as_stream(stream));
 }
 

@@ -14,7 +14,7 @@ struct SearchArgs {
     uint64_t qstride, nq;
     uint32_t qbytes;
     // reference mode
-    uint32_t sp[32];
+    uint32_t sp[64]; // start points: reference mode (<= 32) or FAST seeds (<= 64)
     uint32_t sp_num;
     uint32_t k, L, dlim, maxv, H;
     double cut;
@@ -24,6 +24,7 @@ struct SearchArgs {
     // fast mode
     const uint32_t *Ls, *ks;
     uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;
+    uint32_t nseeds; // FAST: > 1 seeds in sp[] replace the single entry
     uint32_t *counter;
     // outputs
     uint32_t *out_ids;
@@ -41,11 +42,12 @@ __device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
 }
 
 // FAST-mode launch (search.hip); counter is the launch's own work counter
+// seeds: nseeds > 1 distinct start nodes (host array) instead of entry
 int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
-                uint32_t entry, uint32_t L, uint32_t k, const uint32_t *d_L, const uint32_t *d_k,
-                uint32_t Lmax, uint32_t out_k, uint64_t limit, const nbpg_fast_config *cfg,
-                uint32_t *d_ids, float *d_dists, uint32_t *d_stats, uint32_t *counter,
-                hipStream_t stream);
+                uint32_t entry, const uint32_t *seeds, uint32_t nseeds, uint32_t L, uint32_t k,
+                const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
+                uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
+                uint32_t *d_stats, uint32_t *counter, hipStream_t stream);
 
 int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
                       dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a);

@@ -18,9 +18,14 @@
 
 namespace nbpg {
 
+// lane i <- lane i-1 (lane 0 <- 0) over the whole wavefront: the GFX9 DPP
+// wave_shr:1 move, a VALU op, instead of an LDS-routed ds_bpermute
 __device__ __forceinline__ uint64_t shfl_up1(uint64_t v)
 {
-    const uint32_t lo = __shfl_up((uint32_t)v, 1), hi = __shfl_up((uint32_t)(v >> 32), 1);
+    constexpr int WAVE_SHR1 = 0x138;
+    const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, WAVE_SHR1, 0xf, 0xf, false);
+    const uint32_t hi =
+        __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), WAVE_SHR1, 0xf, 0xf, false);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -67,7 +72,10 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x / WAVE;
     const uint32_t FS = 1u << a.flog2;
-    uint32_t *filt = reinterpret_cast<uint32_t *>(lds_filter) + (uint64_t)wv * FS;
+    // per wave: the visited filter, then the query row (16 * NCH bytes)
+    uint32_t *filt = reinterpret_cast<uint32_t *>(lds_filter + (uint64_t)wv * a.wave_bytes);
+    uint32_t *qw = filt + FS;
+    const uint4 *ql = reinterpret_cast<const uint4 *>(qw);
     const uint32_t R = a.R;
     constexpr uint64_t EMPTY = ~0ull;
 
@@ -82,47 +90,70 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         L = __builtin_amdgcn_readfirstlane(max(1u, min(L, (uint32_t)(WAVE * E))));
         uint32_t k = a.ks ? a.ks[qi] : a.k;
         k = __builtin_amdgcn_readfirstlane(min(k, min(L, a.out_k)));
-        uint4 qv[NCH];
+        // the query row into this wave's LDS: one word per lane with a
+        // per-lane bound; distances read it back as broadcast chunks
+        uint32_t nqn;
         {
             const uint8_t *qp = a.queries + (uint64_t)qi * a.qstride;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                uint32_t w[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    uint32_t v = 0;
-                    const uint32_t off = c * 16 + t * 4;
-                    if (off + 4 <= a.qbytes)
-                        v = *reinterpret_cast<const uint32_t *>(qp + off);
-                    else
-                        for (uint32_t bb = 0; bb < 4; ++bb)
-                            if (off + bb < a.qbytes)
-                                v |= (uint32_t)qp[off + bb] << (8 * bb);
-                    w[t] = v;
-                }
-                qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
+            if (lane < 4 * NCH) {
+                const uint32_t off = lane * 4;
+                uint32_t v = 0;
+                if (off + 4 <= a.qbytes)
+                    v = *reinterpret_cast<const uint32_t *>(qp + off);
+                else
+                    for (uint32_t bb = 0; off + bb < a.qbytes && bb < 4; ++bb)
+                        v |= (uint32_t)qp[off + bb] << (8 * bb);
+                qw[lane] = v;
             }
+            wave_sync();
+            uint4 qv[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                qv[c] = ql[c];
+            nqn = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
         }
-        const uint32_t nqn = __builtin_amdgcn_readfirstlane(norm_regs<DT, NCH>(qv));
-        uint32_t qs[4 * NCH];
-        to_uniform<NCH>(qv, qs);
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
         uint64_t b[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
             b[e] = EMPTY;
-        {
+        uint32_t cmps = 1;
+        wave_sync(); // filter cleared before the start nodes are marked
+        if (a.nseeds > 1) {
+            // seeds: one distance per lane, then lane r takes the seed of rank r
+            uint32_t sid = NBPG_SENT;
+            for (uint32_t j = 0; j < a.nseeds; ++j)
+                sid = lane == j ? a.sp[j] : sid;
+            uint64_t key = EMPTY;
+            if (lane < a.nseeds) {
+                key = make_key(dist_row_lds<DT, NCH>(
+                                   reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)sid * a.stride),
+                                   ql, nqn),
+                               sid);
+                filt[fast_slot(sid, a.flog2)] = sid;
+            }
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < a.nseeds; ++j)
+                rank += readlane64(key, (int)j) < key ? 1u : 0u;
+            // lanes past nseeds all rank nseeds; lane `rank` learns who holds that rank
+            const uint32_t src = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src * 4), (int)(uint32_t)key);
+            const uint32_t hi =
+                (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src * 4), (int)(uint32_t)(key >> 32));
+            b[0] = (lane < a.nseeds && lane < L) ? (((uint64_t)hi << 32) | lo) : EMPTY;
+            cmps = a.nseeds;
+        } else {
             const uint32_t en = a.entry;
-            const float d0 = dist_row<DT, NCH>(
-                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), qs, nqn);
+            const float d0 = dist_row_lds<DT, NCH>(
+                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), ql, nqn);
             wave_sync();
             if (lane == 0) {
                 b[0] = make_key(d0, en);
                 filt[fast_slot(en, a.flog2)] = en;
             }
         }
-        uint32_t nexp = 0, cmps = 1;
+        uint32_t nexp = 0;
         wave_sync();
         while (nexp < a.limit) {
             // the W best unexpanded entries
@@ -179,9 +210,9 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
                 key[t] = EMPTY;
                 if (fresh[t])
                     key[t] = make_key(
-                        dist_row<DT, NCH>(
+                        dist_row_lds<DT, NCH>(
                             reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride),
-                            qs, nqn),
+                            ql, nqn),
                         ids[t]);
                 cmps += (uint32_t)__popcll(ballot(fresh[t]));
             }

@@ -141,8 +141,9 @@ static int launch_slot(nbpg_server *sv, Slot &s)
     HIP_TRY(hipMemcpyAsync(s.dq, s.hq, (size_t)off * sv->stride, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dL, s.hL, off * 4ull, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dk, s.hk, off * 4ull, hipMemcpyHostToDevice, s.stream));
-    NBPG_TRY(search_fast(sv->ix, s.dq, off, sv->stride, sv->cfg.entry, 0, 0, s.dL, s.dk, Lmax, kmax,
-                         sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr, s.counter, s.stream));
+    NBPG_TRY(search_fast(sv->ix, s.dq, off, sv->stride, sv->cfg.entry, nullptr, 0, 0, 0, s.dL,
+                         s.dk, Lmax, kmax, sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr,
+                         s.counter, s.stream));
     HIP_TRY(hipMemcpyAsync(s.hids, s.dids, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipMemcpyAsync(s.hd, s.dd, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipEventRecord(s.ev, s.stream));

@@ -401,6 +401,42 @@ def test_fast_and_reference_modes_agree_mostly(gpu, small):
     ix.close()
 
 
+@pytest.mark.parametrize("L,nseeds", [(16, 64), (48, 20), (130, 64), (10, 2)])
+def test_fast_search_seeds(gpu, small, L, nseeds):
+    """FAST search from several start nodes: exact sorted distances, the
+    seeds' own distances counted, recall at least that of the single entry,
+    and the argument checks (repeats, more than 64, row-split kernel)."""
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    seeds = N.pick_seeds(N.to_device(base), nseeds)
+    assert len(seeds) == len(np.unique(seeds)) and len(seeds) <= nseeds
+    gt, gd = O.compgt(base, q, 100, O.U8, tie_mode=1)
+    ids, d, st = N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=seeds)
+    ids, d, st = ids.cpu().numpy().view(np.uint32), d.cpu().numpy(), st.cpu().numpy()
+    assert (np.diff(d, axis=1) >= 0).all()
+    for j in range(0, len(q), 9):
+        ref = np.array([O.distance(O.U8, base[i], q[j]) for i in ids[j]], np.float32)
+        assert (ref == d[j]).all()
+    assert (st[:, 1] >= len(seeds)).all()
+    if L <= 16 and len(seeds) >= L:  # the seeds alone fill the beam
+        for j in range(len(q)):
+            sd = sorted((O.distance(O.U8, base[i], q[j]), i) for i in seeds)
+            assert d[j, 0] <= sd[0][0]
+    e0, _, _ = N.search(ix, qd, L, 10, mode=N.SEARCH_FAST)
+    r_seed = O.recall(ids, gt, base, q, O.U8, 10)
+    r_e0 = O.recall(e0.cpu().numpy().view(np.uint32), gt, base, q, O.U8, 10)
+    assert r_seed >= r_e0 - 0.01, (r_seed, r_e0)
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=[1, 2, 2])
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=list(range(65)))
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=[1, 2, 3], fast=(2, 1, 11, 0, 1))
+    ix.close()
+
+
 @pytest.mark.parametrize("L,cfg", [(300, (2, 1, 10, 0)), (200, (4, 2, 11, 0)), (100, (2, 4, 9, 0)),
                                    (200, (2, 1, 11, 0, 1)), (100, (4, 4, 10, 0, 1))])
 def test_search_fast_large_beams(gpu, small, L, cfg):
