@@ -57,7 +57,34 @@ def build_probe():
     return PROBE_LIB
 
 
+TOOLS = {"Vamana": "vamana.cpp", "CompGT": "compgt.cpp", "PCACompress": "pcacompress.cpp"}
+BIN = os.path.join(HERE, "bin")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_tools():
+    """The host command-line tools (reference CLI flags and file formats) over
+    libnbpg_amd.so: plain host C++ against the HIP runtime, found next to the
+    library through the run path."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for name, src in TOOLS.items():
+        exe = os.path.join(BIN, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+               "-I" + os.path.join(HERE, "..", "include"), "-I" + os.path.join(ROCM, "include"),
+               os.path.join(HERE, "tools", src), "-o", exe + ".tmp",
+               "-L" + HERE, "-lnbpg_amd", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN/..:" + os.path.join(ROCM, "lib")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("tool %s failed:\n%s" % (name, r.stderr[-4000:]))
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build())
     print(build_probe())
+    print(build_tools())
     sys.exit(0)

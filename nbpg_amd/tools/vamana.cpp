@@ -1,0 +1,342 @@
+// vamana.cpp -- the reference's Vamana command line (Vamana.cpp:14-340) over the
+// C-ABI: build (or load) the graph on the device, save <index_path>.graph,
+// then, given queries and ground truth, the recall / QPS sweep of
+// check_nn_recall.h:194-268 with its CSV (check_nn_recall.h:12-48).
+//
+//   Vamana --quant_base_path B.{u8bin,i8bin,u4bin,i4bin} --index_path P [--rebuild]
+//          [--pre_index_path P0] [--R 50] [--L 70] [--VL 140] [--limit 0] [--top_degree 0]
+//          [--num_passes 2] [--num_split 4] [--dynamic_prune] [--rand_sp]
+//          [--min_alpha 1] [--max_alpha 1.1] [--min_cos_angle 0] [--max_cos_angle 0]
+//          [--quant_query_path Q --gt_path G.gt [--res_path R.csv] [--k 10] [--Q 0]
+//           [--q_num 0] [--verbose]]
+//          [--search_mode ref|fast] [--sp_seed 1]
+//
+// --search_mode ref (the default) returns the reference's IDs and counters;
+// fast runs the persistent kernel.  4-split builds draw their random start
+// points from --sp_seed instead of std::random_device.  Not on this path
+// (refused): --use_mips, --dir_bias_scale > 0, rerank over --base_path /
+// --query_path rows that differ from the quantized ones.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <iomanip>
+#include <iostream>
+#include <set>
+#include <sstream>
+#include <sys/stat.h>
+
+#include "tool_io.h"
+
+using namespace nbpg_tools;
+
+namespace {
+
+bool file_exists(const std::string &p)
+{
+    struct stat st;
+    return stat(p.c_str(), &st) == 0;
+}
+
+// point.h:35-120 on the host, for the tie extension of the recall
+float host_dist(int dtype, const uint8_t *a, const uint8_t *b, uint32_t dims)
+{
+    int64_t s = 0;
+    for (uint32_t i = 0; i < dims; ++i) {
+        if (dtype == NBPG_U8) {
+            const int d = (int)a[i] - (int)b[i];
+            s += d * d;
+        } else if (dtype == NBPG_I8) {
+            const int d = (int)(int8_t)a[i] - (int)(int8_t)b[i];
+            s += d * d;
+        } else {
+            const int dh = (int)(a[i] & 0xF0) - (int)(b[i] & 0xF0);
+            const int dl = (int)((a[i] << 4) & 0xFF) - (int)((b[i] << 4) & 0xFF);
+            s += dh * dh + dl * dl;
+        }
+    }
+    return (float)s;
+}
+
+struct Result {
+    double recall;
+    uint32_t avg_cmps, tail_cmps, avg_visited, tail_visited;
+    float qps;
+    uint32_t k, Q;
+    float cut;
+    uint64_t limit;
+    uint32_t degree_limit;
+    uint64_t nq;
+    void print() const
+    {
+        std::cout << "For " << k << "@" << k << " recall = " << recall << ", QPS = " << qps
+                  << ", Q = " << Q << ", cut = " << cut << ", visited limit = " << limit
+                  << ", degree limit: " << degree_limit << ", average visited = " << avg_visited
+                  << ", average cmps = " << avg_cmps << std::endl;
+    }
+};
+
+// query_params.h:66-76
+std::pair<uint32_t, uint32_t> statistics(std::vector<uint32_t> s)
+{
+    std::sort(s.begin(), s.end());
+    double sum = 0;
+    for (auto v : s)
+        sum += v;
+    const uint32_t avg = (uint32_t)(sum / s.size());
+    const size_t tail = (size_t)(.99 * (float)s.size());
+    return {avg, s[tail]};
+}
+
+std::string csv_str(const std::string &v)
+{
+    std::string o = "\"";
+    for (char c : v) {
+        if (c == '"')
+            o += '"';
+        o += c;
+    }
+    return o + "\"";
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    try {
+        Flags f(argc, argv);
+        f.check({"quant_base_path", "pre_index_path", "index_path", "R", "L", "limit", "VL",
+                 "top_degree", "num_passes", "num_split", "dir_bias_scale", "dynamic_prune",
+                 "use_mips", "rebuild", "rand_sp", "max_alpha", "min_alpha", "max_cos_angle",
+                 "min_cos_angle", "cache_raw", "base_path", "query_path", "quant_query_path",
+                 "gt_path", "res_path", "k", "verbose", "Q", "rerank_factor", "q_num",
+                 "search_mode", "sp_seed"});
+        const std::string bp = f.str("quant_base_path"), ip = f.str("index_path");
+        if (bp.empty() || ip.empty())
+            throw std::runtime_error("--quant_base_path and --index_path are required");
+        if (f.has("use_mips"))
+            throw std::runtime_error("--use_mips: inner-product graphs are not on this path");
+        if (f.num("dir_bias_scale", 0) > 0)
+            throw std::runtime_error("--dir_bias_scale > 0 is not on this path");
+        int dtype;
+        if (ends_with(bp, ".u8bin"))
+            dtype = NBPG_U8;
+        else if (ends_with(bp, ".i8bin"))
+            dtype = NBPG_I8;
+        else if (ends_with(bp, ".u4bin"))
+            dtype = NBPG_U4X2;
+        else if (ends_with(bp, ".i4bin"))
+            dtype = NBPG_I4X2;
+        else
+            throw std::runtime_error("--quant_base_path must be .u8bin, .i8bin, .u4bin or .i4bin");
+        nbpg_ok(nbpg_init(0), "nbpg_init");
+        Rows base = read_rows(bp);
+        const uint32_t R = (uint32_t)f.num("R", 50);
+        nbpg_build_params p;
+        p.R = R;
+        p.L = (uint32_t)f.num("L", 70);
+        p.VL = (uint32_t)f.num("VL", 140);
+        p.top_degree = (uint32_t)f.num("top_degree", 0);
+        p.num_split = (uint32_t)f.num("num_split", 4);
+        p.num_passes = (uint32_t)f.num("num_passes", 2);
+        p.limit = (uint64_t)f.num("limit", 0);
+        p.min_alpha = f.num("min_alpha", 1.0);
+        p.max_alpha = f.num("max_alpha", 1.1);
+        p.min_cos_angle = f.num("min_cos_angle", 0.0);
+        p.max_cos_angle = f.num("max_cos_angle", 0.0);
+        p.dynamic_prune = f.has("dynamic_prune");
+        p.rand_sp = f.has("rand_sp");
+        p.repair_isolate = 1;
+        p.sp_seed = (uint64_t)f.num("sp_seed", 1);
+
+        const std::string gfile = ip + ".graph";
+        nbpg_index *ix = nullptr;
+        uint64_t n = base.n;
+        uint32_t gR = R;
+        std::vector<uint32_t> rows;
+        double build_s = 0;
+        uint64_t isolated = 0;
+        if (file_exists(gfile) && !f.has("rebuild")) {
+            rows = read_graph(gfile, n, gR);
+            if (n != base.n)
+                throw std::runtime_error("graph and base sizes differ");
+            nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, gR), "nbpg_index_create");
+            nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
+            nbpg_ok(nbpg_index_set_graph(ix, rows.data(), 0, n), "set_graph");
+        } else {
+            nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, R), "nbpg_index_create");
+            nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
+            if (!f.str("pre_index_path").empty()) { // Vamana.cpp:174-177
+                uint64_t pn = 0;
+                uint32_t pR = 0;
+                auto pre = read_graph(f.str("pre_index_path") + ".graph", pn, pR);
+                if (pn != n || pR != R)
+                    throw std::runtime_error("pre-built graph shape differs");
+                nbpg_ok(nbpg_index_set_graph(ix, pre.data(), 0, n), "set_graph");
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            nbpg_ok(nbpg_build(ix, &p, &isolated), "nbpg_build");
+            build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            rows.assign((size_t)n * (R + 1), 0);
+            nbpg_ok(nbpg_index_get_graph(ix, rows.data(), 0, n), "get_graph");
+            std::cout << "Writing graph with " << n << " points and max degree " << R << std::endl;
+            write_graph(gfile, rows, n, R);
+        }
+        // graph statistics (parse_results.h:7-23, Vamana.cpp:91-111)
+        uint64_t sum = 0;
+        uint32_t maxd = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t d = rows[i * (gR + 1)];
+            sum += d;
+            maxd = std::max(maxd, d);
+        }
+        const double avg_deg = (double)sum / (double)n;
+        std::ostringstream params;
+        params << "R = " << gR << ", L = " << p.L << ", alpha = (" << std::fixed << std::setprecision(2)
+               << p.min_alpha << "," << p.max_alpha << "), use_mips = 0";
+        std::cout << "Vamana graph built with " << n << " points and parameters " << params.str()
+                  << std::endl;
+        std::cout << "Graph has average degree " << avg_deg << ", popular average degree 0"
+                  << " and maximum degree " << maxd << std::endl;
+        std::cout << "Isolated nodes num is " << isolated << std::endl;
+        std::cout << "Graph built in " << build_s << " seconds" << std::endl;
+
+        const std::string qp = f.str("quant_query_path"), gp = f.str("gt_path");
+        if (qp.empty() || gp.empty()) {
+            nbpg_index_destroy(ix);
+            return 0;
+        }
+        if ((!f.str("base_path").empty() && f.str("base_path") != bp) ||
+            (!f.str("query_path").empty() && f.str("query_path") != qp))
+            throw std::runtime_error("rerank over separate raw rows is not on this path");
+        Rows q = read_rows(qp);
+        const uint64_t qn = f.num("q_num", 0) > 0 ? std::min<uint64_t>(q.n, (uint64_t)f.num("q_num", 0)) : q.n;
+        uint32_t gnq = 0, gk = 0;
+        std::vector<uint32_t> gt;
+        read_gt(gp, gnq, gk, gt);
+        if (gnq < qn)
+            throw std::runtime_error("ground truth has fewer queries");
+        const uint32_t kk = f.num("k", 0) > 0 ? (uint32_t)f.num("k", 0) : 10;
+        if (kk > gk)
+            throw std::runtime_error("k too large for the ground truth");
+        const bool verbose = f.has("verbose");
+        const std::string sm = f.str("search_mode", "ref");
+        if (sm != "ref" && sm != "fast")
+            throw std::runtime_error("--search_mode ref|fast");
+        const int mode = sm == "ref" ? NBPG_SEARCH_REF : NBPG_SEARCH_FAST;
+        DevBuf dq(q.data.data(), qn * q.row_bytes());
+
+        auto check = [&](uint32_t k, uint32_t Q, double cut, uint64_t limit, uint32_t dlim) -> Result {
+            DevBuf di(qn * k * 4), dd(qn * k * 4), ds(qn * 8);
+            nbpg_query_params qpm = {k, cut, Q, limit, dlim};
+            const uint32_t sp = 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            nbpg_ok(nbpg_search(ix, dq.p, qn, q.row_bytes(), &sp, 1, &qpm, mode, nullptr,
+                                (uint32_t *)di.p, (float *)dd.p, (uint32_t *)ds.p, nullptr),
+                    "nbpg_search");
+            hip_ok(hipDeviceSynchronize(), "search");
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::vector<uint32_t> ids(qn * k), st(qn * 2);
+            di.download(ids.data(), ids.size() * 4);
+            ds.download(st.data(), st.size() * 4);
+            // recall with ties (check_nn_recall.h:126-167)
+            uint64_t correct = 0;
+            for (uint64_t i = 0; i < qn; ++i) {
+                const uint8_t *qrow = q.data.data() + i * q.row_bytes();
+                std::vector<uint32_t> with_ties(gt.begin() + i * gk, gt.begin() + i * gk + k);
+                const float last = host_dist(dtype, base.data.data() + (uint64_t)gt[i * gk + k - 1] * base.row_bytes(), qrow, base.dims);
+                for (uint32_t l = k; l < gk; ++l) {
+                    const uint32_t g = gt[i * gk + l];
+                    if (host_dist(dtype, base.data.data() + (uint64_t)g * base.row_bytes(), qrow, base.dims) == last)
+                        with_ties.push_back(g);
+                    else
+                        break;
+                }
+                std::set<uint32_t> rep;
+                for (uint32_t l = 0; l < k; ++l)
+                    if (ids[i * k + l] != 0xFFFFFFFFu)
+                        rep.insert(ids[i * k + l]);
+                for (uint32_t g : with_ties)
+                    correct += rep.count(g);
+            }
+            std::vector<uint32_t> vis(qn), cmp(qn);
+            for (uint64_t i = 0; i < qn; ++i) {
+                vis[i] = st[2 * i];
+                cmp[i] = st[2 * i + 1];
+            }
+            auto dsx = statistics(cmp), vsx = statistics(vis);
+            Result r{(double)correct / (double)(k * qn), dsx.first, dsx.second, vsx.first, vsx.second,
+                     (float)(qn / t), k, Q, (float)cut, limit, dlim, qn};
+            if (verbose)
+                std::cout << "search: Q=" << Q << ", k=" << k << ", limit=" << limit << ", recall="
+                          << r.recall << ", visited=" << r.avg_visited << ", comparisons="
+                          << r.avg_cmps << ", QPS=" << r.qps << std::endl;
+            return r;
+        };
+
+        const uint32_t fixedQ = (uint32_t)f.num("Q", 0);
+        if (fixedQ) {
+            for (int i = 0; i < 5; ++i)
+                check(kk, fixedQ, 1.35, n, gR).print();
+            nbpg_index_destroy(ix);
+            return 0;
+        }
+        // check_nn_recall.h:206-257
+        std::vector<Result> results;
+        const uint32_t beams[] = {10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 26, 28,
+                                  30, 32, 34, 36, 38, 40, 45, 50, 55, 60, 65, 70, 80, 90, 100,
+                                  120, 140, 160, 180, 200, 225, 250, 275, 300, 375, 500, 750, 1000};
+        for (uint32_t Q : beams)
+            if (Q >= kk)
+                results.push_back(check(kk, Q, 1.35, n, gR));
+        for (uint32_t l : {10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 28, 30, 35})
+            results.push_back(check(kk, std::max(l, kk), 1.35, l, std::min(gR, 5 * l)));
+        if (100 <= gk)
+            results.push_back(check(100, 1000, 10.0, n, gR));
+        const float buckets[] = {.1f, .2f, .3f, .4f, .5f, .6f, .7f, .75f, .8f, .85f, .9f, .93f, .95f,
+                                 .97f, .98f, .99f, .995f, .999f, .9995f, .9999f, .99995f, .99999f};
+        const int nb = sizeof(buckets) / sizeof(buckets[0]);
+        std::vector<Result> picked;
+        std::vector<float> pb;
+        for (int i = 0; i < nb; ++i) { // parse_results.h:140-178
+            std::vector<Result> cand;
+            for (auto &r : results)
+                if (r.recall >= buckets[i] && (i == nb - 1 || r.recall <= buckets[i + 1]))
+                    cand.push_back(r);
+            if (cand.empty())
+                continue;
+            auto best = *std::max_element(cand.begin(), cand.end(),
+                                          [](const Result &a, const Result &b) { return a.qps < b.qps; });
+            best.print();
+            picked.push_back(best);
+            pb.push_back(buckets[i]);
+        }
+        std::cout << std::endl;
+        const std::string rp = f.str("res_path");
+        if (!rp.empty()) { // check_nn_recall.h:12-48, appended
+            std::ofstream csv(rp, std::ios::app);
+            csv << csv_str("GRAPH") << "," << csv_str("Parameters") << "," << csv_str("Size") << ","
+                << csv_str("Build time") << "," << csv_str("Avg degree") << "," << csv_str("Max degree")
+                << "," << csv_str("Isolated") << "," << csv_str("Popular Avg degree") << std::endl;
+            csv << csv_str("Vamana") << "," << csv_str(params.str()) << "," << n << "," << build_s << ","
+                << avg_deg << "," << maxd << "," << isolated << "," << 0 << std::endl;
+            csv << std::endl;
+            csv << csv_str("Num queries") << "," << csv_str("Target recall") << ","
+                << csv_str("Actual recall") << "," << csv_str("QPS") << "," << csv_str("Average Cmps")
+                << "," << csv_str("Tail Cmps") << "," << csv_str("Average Visited") << ","
+                << csv_str("Tail Visited") << "," << csv_str("k") << "," << csv_str("Q") << ","
+                << csv_str("cut") << std::endl;
+            for (size_t i = 0; i < picked.size(); ++i) {
+                const Result &r = picked[i];
+                csv << r.nq << "," << pb[i] << "," << r.recall << "," << r.qps << "," << r.avg_cmps << ","
+                    << r.tail_cmps << "," << r.avg_visited << "," << r.tail_visited << "," << r.k << ","
+                    << r.Q << "," << r.cut << std::endl;
+            }
+            csv << std::endl << std::endl;
+        }
+        nbpg_index_destroy(ix);
+        return 0;
+    } catch (const std::exception &e) {
+        std::cerr << "Vamana: " << e.what() << std::endl;
+        return 1;
+    }
+}

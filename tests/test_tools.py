@@ -1,0 +1,131 @@
+"""The command-line tools: flag handling on the host; on the device, their
+files equal the reference binaries' own outputs (golden fixtures)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+BIN = os.path.join(ROOT, "nbpg_amd", "bin")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def tool(name):
+    p = os.path.join(BIN, name)
+    if not os.path.exists(p):
+        pytest.skip("tools not built")
+    return p
+
+
+def run(args, **kw):
+    return subprocess.run(args, capture_output=True, text=True, timeout=120, **kw)
+
+
+def write_bin(path, a):
+    with open(path, "wb") as f:
+        np.array([a.shape[0], a.shape[1]], np.uint32).tofile(f)
+        np.ascontiguousarray(a).tofile(f)
+
+
+def read_graph(path):
+    raw = np.fromfile(path, dtype=np.uint32)
+    n, R = int(raw[0]), int(raw[1])
+    deg = raw[2:2 + n]
+    rows = np.zeros((n, R + 1), np.uint32)
+    rows[:, 0] = deg
+    off = 2 + n
+    for i in range(n):
+        rows[i, 1:1 + deg[i]] = raw[off:off + deg[i]]
+        off += deg[i]
+    assert off == len(raw)
+    return rows
+
+
+def live(rows):
+    out = rows.copy()
+    out[:, 1:][np.arange(rows.shape[1] - 1)[None, :] >= rows[:, :1]] = 0
+    return out
+
+
+@pytest.mark.parametrize("name", ["Vamana", "CompGT", "PCACompress"])
+def test_unknown_and_missing_flags_are_errors(name):
+    r = run([tool(name), "--no_such_flag", "3"])
+    assert r.returncode == 1 and "unknown option --no_such_flag" in r.stderr
+    r = run([tool(name)])
+    assert r.returncode == 1 and "required" in r.stderr
+
+
+def test_refused_options():
+    r = run([tool("CompGT"), "--base_path", "b.fbin", "--query_path", "q.fbin", "--gt_path", "g.gt"])
+    assert r.returncode == 1 and "float rows" in r.stderr
+    r = run([tool("Vamana"), "--quant_base_path", "b.u8bin", "--index_path", "g", "--use_mips"])
+    assert r.returncode == 1 and "use_mips" in r.stderr
+
+
+@pytest.mark.gpu
+def test_compgt_tool_writes_the_reference_gt(gpu, golden, tmp_path):
+    write_bin(tmp_path / "b.u8bin", golden["base"])
+    write_bin(tmp_path / "q.u8bin", golden["queries"])
+    for engine in ("mfma", "plain"):
+        gt = tmp_path / ("g_%s.gt" % engine)
+        r = run([tool("CompGT"), "--base_path", str(tmp_path / "b.u8bin"), "--query_path",
+                 str(tmp_path / "q.u8bin"), "--gt_path", str(gt), "--k", "50", "--engine", engine])
+        assert r.returncode == 0, r.stderr
+        raw = np.fromfile(gt, dtype=np.uint32)
+        nq, k = raw[0], raw[1]
+        assert (nq, k) == (40, 50)
+        assert (raw[2:2 + nq * k].reshape(nq, k) == golden["gt_ids"]).all()
+        assert (raw[2 + nq * k:].view(np.float32).reshape(nq, k) == golden["gt_dists"]).all()
+
+
+@pytest.mark.gpu
+def test_vamana_tool_builds_the_reference_graph(gpu, golden, tmp_path):
+    write_bin(tmp_path / "b.u8bin", golden["base"])
+    write_bin(tmp_path / "q.u8bin", golden["queries"])
+    r = run([tool("Vamana"), "--quant_base_path", str(tmp_path / "b.u8bin"), "--index_path",
+             str(tmp_path / "g1"), "--R", "24", "--L", "48", "--max_alpha", "1.2", "--num_split",
+             "0", "--num_passes", "1", "--rebuild"])
+    assert r.returncode == 0, r.stderr
+    assert (live(read_graph(tmp_path / "g1.graph")) == live(golden["graph1"])).all()
+    # the recall sweep over the saved graph (no rebuild), reference-exact search
+    gt = tmp_path / "g.gt"
+    r = run([tool("CompGT"), "--base_path", str(tmp_path / "b.u8bin"), "--query_path",
+             str(tmp_path / "q.u8bin"), "--gt_path", str(gt), "--k", "100"])
+    assert r.returncode == 0, r.stderr
+    csv = tmp_path / "res.csv"
+    r = run([tool("Vamana"), "--quant_base_path", str(tmp_path / "b.u8bin"), "--index_path",
+             str(tmp_path / "g1"), "--R", "24", "--quant_query_path", str(tmp_path / "q.u8bin"),
+             "--gt_path", str(gt), "--res_path", str(csv), "--k", "10"])
+    assert r.returncode == 0, r.stderr
+    assert "For 10@10 recall" in r.stdout
+    text = csv.read_text().splitlines()
+    assert text[0].startswith('"GRAPH","Parameters"') and text[3].startswith('"Num queries"')
+    rows = [ln.split(",") for ln in text[4:] if ln and ln[0].isdigit()]
+    assert rows and all(int(x[0]) == 40 for x in rows)
+
+
+@pytest.mark.gpu
+def test_pcacompress_tool_writes_the_reference_codes(gpu, tmp_path):
+    g = np.load(os.path.join(GOLD, "reference_pca.npz"))
+    for t in "fg":
+        write_bin(tmp_path / ("x_%s.fbin" % t), g["x_" + t])
+        W = g["W_" + t]
+        with open(tmp_path / ("w_%s.mat" % t), "wb") as f:
+            np.array([W.shape[0], W.shape[1]], np.uint32).tofile(f)
+            np.array([bool(g["hasbias_" + t][0])], np.uint8).tofile(f)
+            W.tofile(f)
+            if g["hasbias_" + t][0]:
+                g["b_" + t].tofile(f)
+        for ext in ("fbin", "u8bin", "i8bin", "u4bin", "i4bin"):
+            out = tmp_path / ("y_%s.%s" % (t, ext))
+            r = run([tool("PCACompress"), "--ifile", str(tmp_path / ("x_%s.fbin" % t)), "--ofile",
+                     str(out), "--matFile", str(tmp_path / ("w_%s.mat" % t))])
+            assert r.returncode == 0, r.stderr
+            raw = np.fromfile(out, dtype=np.uint8)
+            n, d = raw[:8].view(np.uint32)
+            want = g["out_%s_%s" % (t, ext)]
+            assert (n, d) == want.shape
+            body = raw[8:]
+            got = body.view(np.float32).reshape(n, d) if ext == "fbin" else body.reshape(n, d)
+            assert (got == want).all(), (t, ext)
