@@ -48,6 +48,8 @@ def main():
                     help="waves_per_group,expand_width,filter_log2,groups,row_split or 'auto'")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a hipGraph")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-entry0", action="store_true",
+                    help="skip timing the FAST path from the single entry node 0 (profiling runs)")
     ap.add_argument("--seeds", type=int, default=64,
                     help="FAST-mode start nodes from k-means over the base (1 = node 0 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -193,7 +195,8 @@ def main():
     L_fast, rec_fast, st_fast = pick(run_fast, "fast")
     sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
     L_plain, rec_plain, st_plain = pick(run_plain, "plain")
-    L_e0, rec_e0, _ = pick(run_fast_entry0, "fast, entry node 0") if seeds is not None \
+    cmp_e0 = seeds is not None and not args.no_entry0
+    L_e0, rec_e0, _ = pick(run_fast_entry0, "fast, entry node 0") if cmp_e0 \
         else (L_fast, rec_fast, None)
 
     def timed(runner, L, steps, warmup, graph):
@@ -232,7 +235,7 @@ def main():
     log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
         (L_fast, qps, np.percentile(per, 50), np.percentile(per, 99)))
     e0_qps = qps
-    if seeds is not None:
+    if cmp_e0:
         w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2, not args.no_graph)
         e0_qps = nq * max(5, args.steps // 2) / w0
         log("fast with the single entry node 0, L=%d: %.0f QPS" % (L_e0, e0_qps))
@@ -275,13 +278,15 @@ def main():
                    "hipgraph": not args.no_graph,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(base.shape[0], nq, base.shape[1], L_fast)},
         "cpu_baseline": cpu,
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
                   "speedup_fast_over_plain": round(qps / plain_qps, 2)},
-        "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)},
+        "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)}
+        if cmp_e0 else None,
         "build_s": round(build_s, 2),
         "groundtruth_s": round(gt_s, 3),
         "groundtruth_valu_s": round(gt_plain_s, 3),
@@ -329,6 +334,21 @@ def gather_ceiling(torch, table_rows=(1 << 20, 3 << 20)):
 
 
 GTD = None
+
+
+def measured_traffic(n, nq, dims, L):
+    """HBM bytes per FAST launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy), or
+    None when this run's workload differs from the profiled one."""
+    path = os.path.join(ROOT, "profiles", "search_reg_kernel_1m_u8_L16.json")
+    try:
+        with open(path) as f:
+            p = json.load(f)
+    except OSError:
+        return None
+    if (p["n"], p["nq"], p["dims"], p["L"]) != (n, nq, dims, L):
+        return None
+    return p["traffic_bytes_per_launch"]
 
 
 def cpu_baseline(base, q, rows, gt, L, seconds):

@@ -102,10 +102,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
     const uint32_t R = a.R;
 
     while (true) {
-        uint32_t qi = 0;
-        if (lane == 0)
-            qi = atomicAdd(a.counter, 1u);
-        qi = __shfl(qi, 0);
+        const uint32_t qi = next_query(a.counter, a.nq, lane);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
@@ -263,7 +260,6 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
     }
 }
 
-__global__ void zero_counter_kernel(uint32_t *c) { c[threadIdx.x] = 0; }
 
 // --------------------------------------------------------------------------
 // host side
@@ -403,7 +399,6 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     uint64_t need = (nq + c.waves_per_group - 1) / c.waves_per_group;
     if (groups > need)
         groups = (uint32_t)std::max<uint64_t>(1, need);
-    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     dim3 g(groups), b(64 * c.waves_per_group);
     if (nseeds > 1 && (!reg || split))
         return set_error(NBPG_EUNSUP, "seeded FAST search needs L <= 256 and row_split = 0");

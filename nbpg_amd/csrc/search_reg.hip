@@ -80,10 +80,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
     constexpr uint64_t EMPTY = ~0ull;
 
     while (true) {
-        uint32_t qi = 0;
-        if (lane == 0)
-            qi = atomicAdd(a.counter, 1u);
-        qi = __builtin_amdgcn_readfirstlane(qi);
+        const uint32_t qi = next_query(a.counter, a.nq, lane);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
@@ -300,10 +297,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
     constexpr uint64_t EMPTY = ~0ull;
 
     while (true) {
-        uint32_t qi = 0;
-        if (lane == 0)
-            qi = atomicAdd(a.counter, 1u);
-        qi = __builtin_amdgcn_readfirstlane(qi);
+        const uint32_t qi = next_query(a.counter, a.nq, lane);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;

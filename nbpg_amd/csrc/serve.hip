@@ -114,6 +114,7 @@ static int alloc_slot(Slot &s, uint32_t cap, uint32_t stride, uint32_t max_k)
     HIP_TRY(hipMalloc((void **)&s.dids, r * 4));
     HIP_TRY(hipMalloc((void **)&s.dd, r * 4));
     HIP_TRY(hipMalloc((void **)&s.counter, 256));
+    HIP_TRY(hipMemset(s.counter, 0, 256)); // work counters start at zero (next_query)
     HIP_TRY(hipHostMalloc((void **)&s.hq, q, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&s.hL, cap * 4ull, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&s.hk, cap * 4ull, hipHostMallocDefault));
