@@ -170,6 +170,19 @@ int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries
                 uint32_t dims, uint32_t k, int mode, uint32_t *d_out_ids, float *d_out_dists,
                 void *stream);
 
+/* ---- rerank: the second half of beam_search_rerank (beamSearch.h:183-236).
+ * d_cand holds each query's frontier from a search over quantized codes
+ * (nq rows of cand_stride ids, first n_cand used, an id >= n ends a row).
+ * The first min(k * rerank_factor, n_cand) candidates (at most 256) are
+ * re-scored against the full-precision rows (raw_dtype NBPG_F32 / NBPG_U8 /
+ * NBPG_I8, dims elements each, packed) and the k nearest by (dist, id) are
+ * written, nq x k; missing slots get NBPG_SENT / +inf.  f32 distances are the
+ * reference's sequential fp32 sum (point.h:137-146). */
+int nbpg_rerank(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+                const void *d_raw_queries, uint64_t nq, const uint32_t *d_cand,
+                uint32_t cand_stride, uint32_t n_cand, uint32_t k, uint32_t rerank_factor,
+                uint32_t *d_out_ids, float *d_out_dists, void *stream);
+
 /* ---- PCACompress (PCACompress.cpp:21-79): y = bias + W x per row
  * (matmul, matrix.h:86-97) and its codes (compress, matrix.h:99-127).
  * x: n rows of d_in elements of x_dtype (NBPG_F32, NBPG_U8 or NBPG_I8; the

@@ -90,6 +90,7 @@ _SIGS = {
     "nbpg_server_wait": (I32, [VP, U64, VP, VP, C.POINTER(DBL)]),
     "nbpg_server_stats": (I32, [VP, I32, C.POINTER(ServerStats)]),
     "nbpg_server_destroy": (I32, [VP]),
+    "nbpg_rerank": (I32, [I32, VP, U64, U32, VP, U64, VP, U32, U32, U32, U32, VP, VP, VP]),
     "nbpg_pca_project": (I32, [I32, VP, U64, U32, VP, VP, U32, I32, I32, VP, VP, VP]),
     "nbpg_pca_quantize": (I32, [VP, U64, U32, I32, VP, VP]),
     "nbpg_pca_stats_f64": (I32, [I32, VP, U64, U32, VP, VP, VP]),
@@ -311,6 +312,21 @@ def compgt(base_dev, q_dev, k, dtype=U8, mode=GT_PLAIN):
     _check(lib().nbpg_compgt(dtype, _ptr(base_dev), base_dev.shape[0], _ptr(q_dev), nq,
                              base_dev.shape[1], k, mode, _ptr(ids), _ptr(d), _stream()),
            "nbpg_compgt")
+    return ids, d
+
+
+def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100):
+    """Re-score each query's frontier (cand_dev: nq x C ids, e.g. the ids of a
+    search over codes with k = L) with the full-precision rows; returns
+    (ids, dists) nq x k."""
+    torch = _torch()
+    nq, C = cand_dev.shape
+    n, dims = raw_base_dev.shape
+    ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+    _check(lib().nbpg_rerank(_x_dtype(raw_base_dev), _ptr(raw_base_dev), n, dims, _ptr(raw_q_dev),
+                             nq, _ptr(cand_dev), C, C, k, rerank_factor, _ptr(ids), _ptr(d),
+                             _stream()), "nbpg_rerank")
     return ids, d
 
 

@@ -1,0 +1,68 @@
+"""Rerank (beam_search_rerank, beamSearch.h:183-236): a FAST search over
+quantized codes, then the frontier re-scored with full-precision rows, equals
+the oracle's distances and (dist, id) order exactly."""
+import numpy as np
+import pytest
+
+import oracle as O
+from nbpg_amd import datagen as D
+
+pytestmark = pytest.mark.gpu
+
+
+def expected(raw, rq, cand, k, factor, dtype):
+    out_i, out_d = [], []
+    for j in range(len(rq)):
+        c = cand[j]
+        valid = np.nonzero(c == 0xFFFFFFFF)[0]
+        m = valid[0] if len(valid) else len(c)
+        m = min(m, k * factor)
+        pts = sorted((O.distance(dtype, raw[i], rq[j]), int(i)) for i in c[:m])[:k]
+        ids = [p[1] for p in pts] + [0xFFFFFFFF] * (k - len(pts))
+        ds = [p[0] for p in pts] + [np.inf] * (k - len(pts))
+        out_i.append(ids)
+        out_d.append(ds)
+    return np.array(out_i, np.uint32), np.array(out_d, np.float32)
+
+
+@pytest.mark.parametrize("raw_kind", ["f32", "u8"])
+def test_rerank_matches_oracle(gpu, raw_kind):
+    N = gpu
+    import torch
+    rng = np.random.default_rng(12)
+    if raw_kind == "f32":
+        raw = (rng.standard_normal((6000, 96)) * np.linspace(2, 0.2, 96)).astype(np.float32)
+        rq = (rng.standard_normal((150, 96)) * np.linspace(2, 0.2, 96)).astype(np.float32)
+        codes = np.clip(np.rint(raw * 20 + 128), 0, 255).astype(np.uint8)
+        qcodes = np.clip(np.rint(rq * 20 + 128), 0, 255).astype(np.uint8)
+        ctype, rdt = N.U8, O.F32
+    else:
+        raw = D.mixture_u8(6000, 64, 30, 25.0, seed=3)
+        rq = D.mixture_u8(150, 64, 30, 25.0, seed=4)
+        codes = ((raw >> 4)[:, 0::2] << 4 | (raw >> 4)[:, 1::2]).astype(np.uint8)  # u4x2
+        qcodes = ((rq >> 4)[:, 0::2] << 4 | (rq >> 4)[:, 1::2]).astype(np.uint8)
+        ctype, rdt = N.U4X2, O.U8
+    ix = N.Index(ctype, len(codes), codes.shape[1], 24)
+    ix.set_points(codes)
+    N.build(ix, 24, 48, alpha=1.2, passes=1, num_split=0)
+    qpad = np.zeros((len(qcodes), ix.row_stride), np.uint8)
+    qpad[:, :qcodes.shape[1]] = qcodes
+    L = 40
+    cand, _, _ = N.search(ix, N.to_device(qpad), L, L, mode=N.SEARCH_FAST)
+    for k, factor in [(10, 3), (10, 100), (1, 5), (32, 1)]:
+        ids, d = N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(), cand, k,
+                          factor)
+        ei, ed = expected(raw, rq, cand.cpu().numpy().view(np.uint32), k, factor, rdt)
+        assert (ids.cpu().numpy().view(np.uint32) == ei).all(), (k, factor)
+        assert (d.cpu().numpy() == ed).all(), (k, factor)
+    # the rerank can only improve the raw-space recall of the frontier head
+    gt, _ = O.compgt(raw, rq, 10, rdt, tie_mode=1) if raw_kind == "u8" else (None, None)
+    if gt is not None:
+        ids, _ = N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(), cand, 10, 4)
+        r_re = O.recall(ids.cpu().numpy().view(np.uint32), gt, raw, rq, O.U8, 10)
+        r_q = O.recall(cand.cpu().numpy().view(np.uint32)[:, :10], gt, raw, rq, O.U8, 10)
+        assert r_re >= r_q
+    with pytest.raises(N.NbpgError):
+        N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(),
+                 torch.zeros((len(rq), 300), dtype=torch.int32, device="cuda"), 100, 3)
+    ix.close()
