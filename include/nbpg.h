@@ -35,11 +35,15 @@ extern "C" {
 #define NBPG_I8 1
 #define NBPG_U4X2 2 /* .u4bin: two 4-bit codes per byte */
 #define NBPG_I4X2 3 /* .i4bin: distance identical to U4X2 (point.h:107-120) */
-#define NBPG_F32 4  /* .fbin / .bin: PCA input rows only (matrix.h:27-57) */
+#define NBPG_F32 4  /* .fbin / .bin: PCA input and raw rerank rows (matrix.h:27-57) */
 
 /* ---- search modes ---- */
 #define NBPG_SEARCH_REF 0  /* one query per workgroup, reference-exact IDs */
 #define NBPG_SEARCH_FAST 1 /* persistent, one wavefront per query, LDS filter + beam */
+/* OR into nbpg_search's mode: write the whole frontier (up to L per query,
+ * output stride L) instead of its first k; the reference mode still cuts
+ * with k.  This is the candidate list the rerank step reads. */
+#define NBPG_SEARCH_FRONTIER 0x10
 
 /* ---- CompGT modes ---- */
 #define NBPG_GT_PLAIN 0 /* VALU dot4, one query tile per workgroup */
@@ -143,6 +147,27 @@ int nbpg_search(const nbpg_index *idx, const void *d_queries, uint64_t nq, uint6
                 const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
                 int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
                 uint32_t *d_out_stats, void *stream);
+/* qsearchAll over quantized rows with full-precision rows beside them
+ * (Base_Points != Q_Base_Points, beam_search_rerank, beamSearch.h:183-247).
+ * The traversal uses the index (quantized) rows and d_queries; the raw rows
+ * (raw_dtype, raw_dims elements each, packed n x raw_dims on the device)
+ * and d_raw_queries (raw_query_stride bytes apart) give the final distances:
+ *   rerank_factor > 0: rerank mode -- the first min(k * rerank_factor, fs)
+ *     frontier entries are re-scored and the k nearest by (dist, id) kept;
+ *   rerank_factor = 0: filtering mode (NBPG_SEARCH_REF only) -- every visited
+ *     entry is re-scored and the first min(k, fs, visited) are kept; its
+ *     dist_cmps counter adds those exact distances as the reference does.
+ * NBPG_SEARCH_REF returns the reference's IDs (f32 rows: distances summed in
+ * index order, point.h:137-146).  NBPG_SEARCH_FAST re-scores the top
+ * min(k * rerank_factor, L) <= 256 of its beam, using the index's workspace
+ * (calls on one index must not overlap).  Outputs nq x k. */
+int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq,
+                       uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                       uint32_t raw_dims, const void *d_raw_queries, uint64_t raw_query_stride,
+                       const uint32_t *h_start_points, uint32_t sp_num,
+                       const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
+                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                       uint32_t *d_out_stats, void *stream);
 /* FAST mode with per-query beam width and k (uneven batches); d_L, d_k are
  * nq u32 each on the device, outputs have stride out_k per query */
 int nbpg_search_uneven(const nbpg_index *idx, const void *d_queries, uint64_t nq,

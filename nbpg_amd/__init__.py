@@ -20,7 +20,7 @@ CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
 # PCACompress picks the code type from the output extension (PCACompress.cpp:25-37)
 CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
                ".i4bin": CODE_I4}
-SEARCH_REF, SEARCH_FAST = 0, 1
+SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER = 0, 1, 0x10
 GT_PLAIN, GT_MFMA, GT_REFTIES = 0, 1, 0x100
 SENT = 0xFFFFFFFF
 DTYPE_OF_EXT = {".u8bin": U8, ".i8bin": I8, ".u4bin": U4X2, ".i4bin": I4X2}
@@ -78,6 +78,8 @@ _SIGS = {
                               C.POINTER(U32)]),
     "nbpg_l2_batch": (I32, [VP, VP, VP, U64, VP, VP]),
     "nbpg_l2_pairs": (I32, [I32, VP, VP, U64, U32, VP, VP]),
+    "nbpg_search_rerank": (I32, [VP, VP, U64, U64, I32, VP, U32, VP, U64, VP, U32,
+                                 C.POINTER(QueryParams), U32, I32, VP, VP, VP, VP, VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
                           C.POINTER(FastConfig), VP, VP, VP, VP]),
     "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, U32, C.POINTER(FastConfig),
@@ -203,17 +205,22 @@ def l2_batch(index, q_dev, ids_dev):
 
 
 def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None,
-           entry=0, start_points=None, fast=None, stats=True, out=None):
-    """Beam search for every row of q_dev (uint8 tensor, nq x stride bytes)."""
+           entry=0, start_points=None, fast=None, stats=True, out=None, frontier=False):
+    """Beam search for every row of q_dev (uint8 tensor, nq x stride bytes).
+    frontier=True returns each query's whole frontier (nq x L) instead of
+    its first k (the reference mode still cuts with k)."""
     torch = _torch()
     nq = q_dev.shape[0]
     qstride = q_dev.shape[1] * q_dev.element_size()
     qp = QueryParams(k, cut, L, index.n if limit is None else limit,
                      index.R if degree_limit is None else degree_limit)
     sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
+    width = L if frontier else k
+    if frontier:
+        mode |= SEARCH_FRONTIER
     if out is None:
-        ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
-        d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        ids = torch.empty((nq, width), dtype=torch.int32, device="cuda")
+        d = torch.empty((nq, width), dtype=torch.float32, device="cuda")
         st = torch.empty((nq, 2), dtype=torch.int32, device="cuda") if stats else None
     else:
         ids, d, st = out
@@ -221,6 +228,36 @@ def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_lim
     _check(lib().nbpg_search(index.h, _ptr(q_dev), nq, qstride, sp.ctypes.data, len(sp),
                              C.byref(qp), mode, C.byref(fc) if fc is not None else None,
                              _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search")
+    return ids, d, st
+
+
+def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100,
+                  mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None, entry=0,
+                  start_points=None, fast=None, raw_dtype=None):
+    """Search over the index's quantized rows with full-precision rows for the
+    final distances (beam_search_rerank): rerank_factor > 0 re-scores the
+    frontier head, 0 is the filtering mode (SEARCH_REF only).  raw_base_dev:
+    n x raw_dims (float32 / uint8 / int8 tensor), raw_q_dev: nq x raw_dims.
+    raw_dtype overrides the type taken from the tensor (U4X2 / I4X2 rows).
+    Returns (ids, dists, stats), nq x k."""
+    torch = _torch()
+    nq = q_dev.shape[0]
+    qstride = q_dev.shape[1] * q_dev.element_size()
+    qp = QueryParams(k, cut, L, index.n if limit is None else limit,
+                     index.R if degree_limit is None else degree_limit)
+    sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
+    ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+    st = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+    fc = FastConfig(*fast) if fast is not None else None
+    raw_dims = raw_base_dev.shape[1]
+    rdt = _x_dtype(raw_base_dev) if raw_dtype is None else raw_dtype
+    _check(lib().nbpg_search_rerank(index.h, _ptr(q_dev), nq, qstride, rdt,
+                                    _ptr(raw_base_dev), raw_dims, _ptr(raw_q_dev),
+                                    raw_q_dev.shape[1] * raw_q_dev.element_size(),
+                                    sp.ctypes.data, len(sp), C.byref(qp), rerank_factor, mode,
+                                    C.byref(fc) if fc is not None else None, _ptr(ids), _ptr(d),
+                                    _ptr(st), _stream()), "nbpg_search_rerank")
     return ids, d, st
 
 
@@ -315,7 +352,7 @@ def compgt(base_dev, q_dev, k, dtype=U8, mode=GT_PLAIN):
     return ids, d
 
 
-def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100):
+def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100, raw_dtype=None):
     """Re-score each query's frontier (cand_dev: nq x C ids, e.g. the ids of a
     search over codes with k = L) with the full-precision rows; returns
     (ids, dists) nq x k."""
@@ -324,7 +361,8 @@ def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100):
     n, dims = raw_base_dev.shape
     ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
     d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
-    _check(lib().nbpg_rerank(_x_dtype(raw_base_dev), _ptr(raw_base_dev), n, dims, _ptr(raw_q_dev),
+    rdt = _x_dtype(raw_base_dev) if raw_dtype is None else raw_dtype
+    _check(lib().nbpg_rerank(rdt, _ptr(raw_base_dev), n, dims, _ptr(raw_q_dev),
                              nq, _ptr(cand_dev), C, C, k, rerank_factor, _ptr(ids), _ptr(d),
                              _stream()), "nbpg_rerank")
     return ids, d
@@ -338,7 +376,7 @@ def _x_dtype(t):
         return U8
     if t.dtype == torch.int8:
         return I8
-    raise NbpgError("PCA input rows must be float32, uint8 or int8, not %s" % t.dtype)
+    raise NbpgError("rows must be float32, uint8 or int8, not %s" % t.dtype)
 
 
 def pca_project(x_dev, W_dev, bias_dev, code=CODE_I8, mode=PCA_BF16, want_y=False):

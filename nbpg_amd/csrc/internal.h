@@ -62,6 +62,13 @@ extern DevInfo g_dev;
 
 int ensure_init();
 int ensure_ws(nbpg_index *idx, size_t bytes);
+// rerank of candidate rows (rerank.hip); raw query rows at q_stride bytes
+#define RERANK_MAX 256
+int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+                  const void *d_raw_queries, uint64_t q_stride, uint64_t nq,
+                  const uint32_t *d_cand, uint32_t cand_stride, uint32_t n_cand, uint32_t k,
+                  uint32_t rerank_factor, uint32_t *d_out_ids, float *d_out_dists,
+                  hipStream_t stream);
 // 16-byte chunks per row for `bytes` of payload: one of 2, 4, 6, 8, 16
 uint32_t nch_for_bytes(uint32_t bytes);
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }

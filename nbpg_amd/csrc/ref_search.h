@@ -102,6 +102,22 @@ __device__ __forceinline__ void wave_rank_sort(const uint64_t *src, uint32_t n, 
     }
 }
 
+// the kk smallest of n distinct keys, in order: dst[rank(src[i])] = src[i]
+// for rank < kk; one wave
+__device__ __forceinline__ void wave_rank_select(const uint64_t *src, uint32_t n, uint32_t kk,
+                                                 uint64_t *dst, uint32_t lane)
+{
+    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
+        const uint32_t i = i0 + lane;
+        const uint64_t v = i < n ? src[i] : 0;
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < n; ++j)
+            r += src[j] < v;
+        if (i < n && r < kk)
+            dst[r] = v;
+    }
+}
+
 // has_been_seen (query_params.h:223-230) applied to the valid lanes of one
 // 64-id chunk as if called lane by lane in order: a lane sees the id the
 // nearest lower lane of its slot wrote, or the slot's prior content.
@@ -164,6 +180,7 @@ struct RefOut {
     uint64_t *F; // final frontier (sorted), fs entries
     uint32_t fs, vs;
     uint64_t nvis, cmps;
+    uint64_t nins; // pops that landed in the capped visited list
 };
 
 struct GraphView {
@@ -207,7 +224,7 @@ __device__ RefOut ref_search(RefState &s, const GraphView &g, const uint32_t (&q
     for (uint32_t t = lane; t < sp_num; t += WAVE)
         s.unv[t] = Fp[t];
     uint32_t fs = sp_num, remain = sp_num, offset = 0, cand_n = 0, vs = 0;
-    uint64_t nvis = 0, cmps = sp_num;
+    uint64_t nvis = 0, cmps = sp_num, nins = 0;
     wave_sync();
 
     while (remain > offset && nvis < limit) {
@@ -247,6 +264,7 @@ __device__ RefOut ref_search(RefState &s, const GraphView &g, const uint32_t (&q
                 wave_sync();
             }
         }
+        nins += pos < nvs;
         if (pos < nvs && lane == 0)
             s.vis[pos] = cur;
         vs = nvs;
@@ -335,6 +353,7 @@ __device__ RefOut ref_search(RefState &s, const GraphView &g, const uint32_t (&q
     o.vs = vs;
     o.nvis = nvis;
     o.cmps = cmps;
+    o.nins = nins;
     return o;
 }
 

@@ -7,67 +7,15 @@
 // candidate's raw row in index order, so a float distance is the reference's
 // sequential fp32 sum (point.h:137-146) bit for bit and an integer one is
 // exact.  The <= 256 keys are then sorted in LDS.
-#include "internal.h"
+#include <algorithm>
+
+#include "raw_rows.h"
 
 namespace nbpg {
 
-#define RR_MAX 256 // candidates re-scored per query
-
-template <int RT> struct raw_row;
-template <> struct raw_row<NBPG_F32> {
-    static __device__ __forceinline__ float dist(const uint8_t *a, const uint8_t *b, uint32_t dims)
-    {
-        const float *p = reinterpret_cast<const float *>(a), *q = reinterpret_cast<const float *>(b);
-        float s = 0.0f;
-        uint32_t i = 0;
-        if ((dims & 3) == 0 && ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0) {
-            // 16-byte loads, the sum still taken element by element in order
-            for (; i < dims; i += 4) {
-                const float4 x = *reinterpret_cast<const float4 *>(p + i);
-                const float4 y = *reinterpret_cast<const float4 *>(q + i);
-                float d = __fsub_rn(x.x, y.x);
-                s = __fadd_rn(s, __fmul_rn(d, d));
-                d = __fsub_rn(x.y, y.y);
-                s = __fadd_rn(s, __fmul_rn(d, d));
-                d = __fsub_rn(x.z, y.z);
-                s = __fadd_rn(s, __fmul_rn(d, d));
-                d = __fsub_rn(x.w, y.w);
-                s = __fadd_rn(s, __fmul_rn(d, d));
-            }
-        }
-        for (; i < dims; ++i) {
-            const float d = __fsub_rn(p[i], q[i]);
-            s = __fadd_rn(s, __fmul_rn(d, d));
-        }
-        return s;
-    }
-};
-template <> struct raw_row<NBPG_U8> {
-    static __device__ __forceinline__ float dist(const uint8_t *a, const uint8_t *b, uint32_t dims)
-    {
-        int32_t s = 0;
-        for (uint32_t i = 0; i < dims; ++i) {
-            const int32_t d = (int32_t)a[i] - (int32_t)b[i];
-            s += d * d;
-        }
-        return (float)s;
-    }
-};
-template <> struct raw_row<NBPG_I8> {
-    static __device__ __forceinline__ float dist(const uint8_t *a, const uint8_t *b, uint32_t dims)
-    {
-        int32_t s = 0;
-        for (uint32_t i = 0; i < dims; ++i) {
-            const int32_t d = (int32_t)(int8_t)a[i] - (int32_t)(int8_t)b[i];
-            s += d * d;
-        }
-        return (float)s;
-    }
-};
-
 struct RerankArgs {
     const uint8_t *base, *queries;
-    uint64_t n, nq, row_bytes;
+    uint64_t n, nq, row_bytes, qstride;
     uint32_t dims, cand_stride, num_check, k;
     const uint32_t *cand;
     uint32_t *ids;
@@ -77,13 +25,13 @@ struct RerankArgs {
 template <int RT>
 __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
 {
-    __shared__ uint64_t keys[4][RR_MAX];
+    __shared__ uint64_t keys[4][RERANK_MAX];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint64_t q = (uint64_t)blockIdx.x * 4 + wv;
     if (q >= a.nq)
         return;
     uint64_t *kw = keys[wv];
-    const uint8_t *qrow = a.queries + q * a.row_bytes;
+    const uint8_t *qrow = a.queries + q * a.qstride;
     // the frontier ends at its first empty slot
     uint32_t m = a.num_check;
     for (uint32_t i = lane; i < a.num_check; i += WAVE) {
@@ -128,6 +76,44 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
     }
 }
 
+int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+                  const void *d_raw_queries, uint64_t q_stride, uint64_t nq,
+                  const uint32_t *d_cand, uint32_t cand_stride, uint32_t n_cand, uint32_t k,
+                  uint32_t rerank_factor, uint32_t *d_out_ids, float *d_out_dists,
+                  hipStream_t s)
+{
+    const uint64_t nc = std::min<uint64_t>((uint64_t)k * rerank_factor, n_cand);
+    if (nc > RERANK_MAX)
+        return set_error(NBPG_EUNSUP, "%llu candidates to re-score (at most %d)",
+                         (unsigned long long)nc, RERANK_MAX);
+    if (nq == 0)
+        return NBPG_OK;
+    RerankArgs a;
+    a.base = (const uint8_t *)d_raw_base;
+    a.queries = (const uint8_t *)d_raw_queries;
+    a.n = n;
+    a.nq = nq;
+    a.row_bytes = (uint64_t)dims * raw_elem_bytes(raw_dtype);
+    a.qstride = q_stride;
+    a.dims = dims;
+    a.cand_stride = cand_stride;
+    a.num_check = (uint32_t)nc;
+    a.k = k;
+    a.cand = d_cand;
+    a.ids = d_out_ids;
+    a.dists = d_out_dists;
+    const dim3 grid((uint32_t)((nq + 3) / 4)), block(256);
+    if (raw_dtype == NBPG_F32)
+        hipLaunchKernelGGL(rerank_kernel<NBPG_F32>, grid, block, 0, s, a);
+    else if (raw_dtype == NBPG_U8)
+        hipLaunchKernelGGL(rerank_kernel<NBPG_U8>, grid, block, 0, s, a);
+    else if (raw_dtype == NBPG_I8)
+        hipLaunchKernelGGL(rerank_kernel<NBPG_I8>, grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL(rerank_kernel<NBPG_U4X2>, grid, block, 0, s, a);
+    return hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "rerank launch");
+}
+
 } // namespace nbpg
 
 using namespace nbpg;
@@ -139,38 +125,13 @@ extern "C" int nbpg_rerank(int raw_dtype, const void *d_raw_base, uint64_t n, ui
                            void *stream)
 {
     NBPG_TRY(ensure_init());
-    if (raw_dtype != NBPG_U8 && raw_dtype != NBPG_I8 && raw_dtype != NBPG_F32)
-        return set_error(NBPG_EINVAL, "raw rows must be u8, i8 or f32");
+    if (raw_dtype < NBPG_U8 || raw_dtype > NBPG_F32)
+        return set_error(NBPG_EINVAL, "raw dtype %d", raw_dtype);
     if (nq && (!d_raw_base || !d_raw_queries || !d_cand || !d_out_ids))
         return set_error(NBPG_EINVAL, "null argument");
     if (k == 0 || rerank_factor == 0 || n_cand == 0 || cand_stride < n_cand || dims == 0)
         return set_error(NBPG_EINVAL, "k, rerank_factor and n_cand must be positive");
-    const uint64_t nc = std::min<uint64_t>((uint64_t)k * rerank_factor, n_cand);
-    if (nc > RR_MAX)
-        return set_error(NBPG_EUNSUP, "%llu candidates to re-score (at most %d)",
-                         (unsigned long long)nc, RR_MAX);
-    if (nq == 0)
-        return NBPG_OK;
-    RerankArgs a;
-    a.base = (const uint8_t *)d_raw_base;
-    a.queries = (const uint8_t *)d_raw_queries;
-    a.n = n;
-    a.nq = nq;
-    a.row_bytes = (uint64_t)dims * (raw_dtype == NBPG_F32 ? 4 : 1);
-    a.dims = dims;
-    a.cand_stride = cand_stride;
-    a.num_check = (uint32_t)nc;
-    a.k = k;
-    a.cand = d_cand;
-    a.ids = d_out_ids;
-    a.dists = d_out_dists;
-    const dim3 grid((uint32_t)((nq + 3) / 4)), block(256);
-    hipStream_t s = as_stream(stream);
-    if (raw_dtype == NBPG_F32)
-        hipLaunchKernelGGL(rerank_kernel<NBPG_F32>, grid, block, 0, s, a);
-    else if (raw_dtype == NBPG_U8)
-        hipLaunchKernelGGL(rerank_kernel<NBPG_U8>, grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL(rerank_kernel<NBPG_I8>, grid, block, 0, s, a);
-    return hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "rerank launch");
+    return rerank_launch(raw_dtype, d_raw_base, n, dims, d_raw_queries,
+                         (uint64_t)dims * raw_elem_bytes(raw_dtype), nq, d_cand, cand_stride,
+                         n_cand, k, rerank_factor, d_out_ids, d_out_dists, as_stream(stream));
 }

@@ -40,13 +40,51 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
         const RefOut o = ref_search<DT, NCH>(
             s, g, qs, nqn, [&](uint32_t t) { return a.sp[t]; }, a.sp_num, a.k, a.cut, a.limit,
             a.dlim, NBPG_SENT, lane);
-        // frontier[0:k] (beamSearch.h:249-254)
-        for (uint32_t j = lane; j < a.k; j += WAVE) {
-            const uint64_t oi = qi * a.k + j;
-            if (j < o.fs) {
-                a.out_ids[oi] = key_id(o.F[j]);
+        const uint64_t *res = o.F;
+        uint32_t nres = o.fs;
+        uint64_t cmps = o.cmps;
+        if (a.exact) {
+            const uint8_t *rq = a.raw_q + qi * a.raw_qstride;
+            uint64_t *src;
+            uint32_t n, kk;
+            if (a.exact == 1) {
+                // rerank: frontier[0:num_check] re-scored (beamSearch.h:221-236)
+                n = (uint32_t)min<uint64_t>((uint64_t)a.k * a.rerank_factor, o.fs);
+                for (uint32_t j = lane; j < n; j += WAVE) {
+                    const uint32_t id = key_id(o.F[j]);
+                    s.cand[j] = make_key(
+                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims), id);
+                }
+                src = s.cand;
+                kk = min(a.k, n);
+            } else {
+                // filtering: the capped visited list carries exact distances
+                // (beamSearch.h:106-113), is re-sorted by them (163-168) and
+                // cut to min(k, fs, vs) (217, 239-246)
+                n = o.vs;
+                for (uint32_t j = lane; j < n; j += WAVE) {
+                    const uint32_t id = key_id(s.vis[j]);
+                    s.vis[j] = make_key(
+                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims), id);
+                }
+                src = s.vis;
+                kk = min(a.k, min(o.fs, o.vs));
+                cmps += o.nins;
+            }
+            wave_sync();
+            wave_rank_select(src, n, kk, s.cs, lane);
+            wave_sync();
+            res = s.cs;
+            nres = kk;
+        }
+        // frontier[0:k] (beamSearch.h:249-254), or the whole frontier when
+        // out_k = L
+        for (uint32_t j = lane; j < a.out_k; j += WAVE) {
+            const uint64_t oi = qi * a.out_k + j;
+            if (j < nres) {
+                a.out_ids[oi] = key_id(res[j]);
                 if (a.out_dists)
-                    a.out_dists[oi] = key_dist(o.F[j]);
+                    a.out_dists[oi] = key_dist(res[j]);
             } else {
                 a.out_ids[oi] = NBPG_SENT;
                 if (a.out_dists)
@@ -55,7 +93,7 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
         }
         if (a.out_stats && lane == 0) {
             a.out_stats[2 * qi] = (uint32_t)o.nvis;
-            a.out_stats[2 * qi + 1] = (uint32_t)o.cmps;
+            a.out_stats[2 * qi + 1] = (uint32_t)cmps;
         }
         wave_sync();
     }
@@ -265,9 +303,19 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
 // host side
 // --------------------------------------------------------------------------
 
+// full-precision rows of a rerank / filtering search
+struct ExactRows {
+    int dtype;
+    const void *base, *queries;
+    uint32_t dims;
+    uint64_t qstride;
+    uint32_t rerank_factor; // 0: filtering mode
+};
+
 static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
                       const uint32_t *sp, uint32_t sp_num, const nbpg_query_params *qp,
-                      uint32_t *d_ids, float *d_dists, uint32_t *d_stats, hipStream_t stream)
+                      uint32_t out_k, const ExactRows *ex, uint32_t *d_ids, float *d_dists,
+                      uint32_t *d_stats, hipStream_t stream)
 {
     SearchArgs a;
     memset(&a, 0, sizeof(a));
@@ -292,9 +340,20 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     uint64_t mv = 8ull * qp->L; // beamSearch.h:279, 290
     a.maxv = (uint32_t)std::min<uint64_t>(mv, 1u << 30);
     a.H = ref_hash_size(qp->L);
+    a.out_k = out_k;
     a.out_ids = d_ids;
     a.out_dists = d_dists;
     a.out_stats = d_stats;
+    if (ex) {
+        a.exact = ex->rerank_factor ? 1 : 2;
+        a.rerank_factor = ex->rerank_factor;
+        a.raw_dtype = (uint32_t)ex->dtype;
+        a.raw_dims = ex->dims;
+        a.raw_row = (uint64_t)ex->dims * raw_elem_bytes(ex->dtype);
+        a.raw_base = (const uint8_t *)ex->base;
+        a.raw_q = (const uint8_t *)ex->queries;
+        a.raw_qstride = ex->qstride;
+    }
     const RefLayout lo = ref_layout(a.L, a.R, a.maxv, a.H);
     a.state_bytes = lo.total;
     const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
@@ -423,13 +482,15 @@ using namespace nbpg;
 
 extern "C" {
 
-int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,
-                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
-                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
-                uint32_t *d_out_stats, void *stream)
+static int check_search(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                        uint64_t query_stride, const uint32_t *h_start_points, uint32_t sp_num,
+                        const nbpg_query_params *qp, int base_mode, bool want_ids,
+                        const uint32_t *d_out_ids)
 {
-    if (!ix || !qp || (nq && (!d_queries || (qp->k && !d_out_ids))))
+    if (!ix || !qp || (nq && (!d_queries || (want_ids && !d_out_ids))))
         return set_error(NBPG_EINVAL, "null argument");
+    if (base_mode != NBPG_SEARCH_REF && base_mode != NBPG_SEARCH_FAST)
+        return set_error(NBPG_EINVAL, "search mode %d", base_mode);
     if (qp->L == 0)
         return set_error(NBPG_EINVAL, "beam size L = 0");
     if (qp->k > qp->L) // beamSearch.h:268-273
@@ -439,25 +500,75 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                          (unsigned long long)query_stride, ix->dims);
     // query_params.h:206-210 for the reference mode; FAST takes up to 64 seeds
     if (sp_num == 0 || !h_start_points ||
-        (mode == NBPG_SEARCH_REF && (sp_num > 32 || sp_num > qp->L)) || sp_num > 64)
+        (base_mode == NBPG_SEARCH_REF && (sp_num > 32 || sp_num > qp->L)) || sp_num > 64)
         return set_error(NBPG_EINVAL, "start points: %u (reference mode 1..min(32, L), fast 1..64)",
                          sp_num);
     for (uint32_t s = 0; s < sp_num; ++s)
         if (h_start_points[s] >= ix->n)
             return set_error(NBPG_EINVAL, "start point %u >= n", h_start_points[s]);
+    if (base_mode == NBPG_SEARCH_FAST && nq >= 0xFFFFFFFFull)
+        return set_error(NBPG_EUNSUP, "nq too large");
+    return NBPG_OK;
+}
+
+int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,
+                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
+                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                uint32_t *d_out_stats, void *stream)
+{
+    const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
+    const int base = mode & ~NBPG_SEARCH_FRONTIER;
+    NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, base,
+                          frontier || qp->k, d_out_ids));
     if (nq == 0)
         return NBPG_OK;
-    if (mode == NBPG_SEARCH_REF)
-        return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, d_out_ids,
-                          d_out_dists, d_out_stats, as_stream(stream));
-    if (mode == NBPG_SEARCH_FAST) {
-        if (nq >= 0xFFFFFFFFull)
-            return set_error(NBPG_EUNSUP, "nq too large");
-        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
-                           sp_num, qp->L, qp->k, nullptr, nullptr, qp->L, qp->k, qp->limit, fast,
-                           d_out_ids, d_out_dists, d_out_stats, ix->counter, as_stream(stream));
+    const uint32_t out_k = frontier ? qp->L : qp->k;
+    if (base == NBPG_SEARCH_REF)
+        return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, out_k,
+                          nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
+    return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points, sp_num,
+                       qp->L, out_k, nullptr, nullptr, qp->L, out_k, qp->limit, fast, d_out_ids,
+                       d_out_dists, d_out_stats, ix->counter, as_stream(stream));
+}
+
+int nbpg_search_rerank(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                       uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                       uint32_t raw_dims, const void *d_raw_queries, uint64_t raw_query_stride,
+                       const uint32_t *h_start_points, uint32_t sp_num,
+                       const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
+                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                       uint32_t *d_out_stats, void *stream)
+{
+    NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode,
+                          qp && qp->k, d_out_ids));
+    if (raw_dtype < NBPG_U8 || raw_dtype > NBPG_F32)
+        return set_error(NBPG_EINVAL, "raw dtype %d", raw_dtype);
+    if (raw_dims == 0 || raw_query_stride < (uint64_t)raw_dims * raw_elem_bytes(raw_dtype))
+        return set_error(NBPG_EINVAL, "raw dims %u / raw query stride %llu", raw_dims,
+                         (unsigned long long)raw_query_stride);
+    if (nq && (!d_raw_base || !d_raw_queries))
+        return set_error(NBPG_EINVAL, "null raw rows");
+    if (mode == NBPG_SEARCH_FAST && rerank_factor == 0)
+        return set_error(NBPG_EUNSUP, "filtering mode (rerank_factor 0) needs NBPG_SEARCH_REF");
+    if (nq == 0 || qp->k == 0)
+        return NBPG_OK;
+    hipStream_t s = as_stream(stream);
+    if (mode == NBPG_SEARCH_REF) {
+        const ExactRows ex = {raw_dtype, d_raw_base, d_raw_queries, raw_dims, raw_query_stride,
+                              rerank_factor};
+        return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, qp->k, &ex,
+                          d_out_ids, d_out_dists, d_out_stats, s);
     }
-    return set_error(NBPG_EINVAL, "search mode %d", mode);
+    // FAST: the top L of each beam into the index's workspace, then re-scored
+    if (std::min<uint64_t>((uint64_t)qp->k * rerank_factor, qp->L) > RERANK_MAX)
+        return set_error(NBPG_EUNSUP, "min(k * rerank_factor, L) > %d", RERANK_MAX);
+    NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), nq * qp->L * sizeof(uint32_t)));
+    uint32_t *cand = (uint32_t *)ix->ws;
+    NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
+                         sp_num, qp->L, qp->L, nullptr, nullptr, qp->L, qp->L, qp->limit, fast,
+                         cand, nullptr, d_out_stats, ix->counter, s));
+    return rerank_launch(raw_dtype, d_raw_base, ix->n, raw_dims, d_raw_queries, raw_query_stride,
+                         nq, cand, qp->L, qp->L, qp->k, rerank_factor, d_out_ids, d_out_dists, s);
 }
 
 int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,

@@ -1,6 +1,6 @@
 // search_args.h -- launch arguments and helpers shared by the search kernels.
 #pragma once
-#include "internal.h"
+#include "raw_rows.h"
 
 namespace nbpg {
 
@@ -21,6 +21,11 @@ struct SearchArgs {
     uint64_t limit;
     uint8_t *gscratch;
     uint64_t state_bytes;
+    // reference mode over full-precision rows (beamSearch.h:183-247):
+    // exact = 1 rerank of frontier[0:min(k * rerank_factor, fs)], 2 filtering
+    uint32_t exact, raw_dtype, raw_dims, rerank_factor;
+    const uint8_t *raw_base, *raw_q;
+    uint64_t raw_row, raw_qstride;
     // fast mode
     const uint32_t *Ls, *ks;
     uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;

@@ -80,6 +80,11 @@ def lib():
         L.nbpgo_search.argtypes = [C.POINTER(Graph), C.POINTER(Points), C.POINTER(Points),
                                    C.c_void_p, C.c_uint32, C.POINTER(QParams), C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int]
+        L.nbpgo_search_exact.restype = C.c_int
+        L.nbpgo_search_exact.argtypes = [C.POINTER(Graph), C.POINTER(Points), C.POINTER(Points),
+                                         C.POINTER(Points), C.POINTER(Points), C.c_void_p,
+                                         C.c_uint32, C.POINTER(QParams), C.c_uint32, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int]
         L.nbpgo_build.restype = C.c_int
         L.nbpgo_build.argtypes = [C.POINTER(Points), C.POINTER(Graph), C.POINTER(BuildParams),
                                   C.POINTER(C.c_uint64), C.c_int]
@@ -122,6 +127,13 @@ def ref():
                                    C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                    C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p,
                                    C.c_void_p]
+        R.nbpgr_search_exact.restype = C.c_int
+        R.nbpgr_search_exact.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32,
+                                         C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                         C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
+                                         C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_void_p]
         R.nbpgr_prune.restype = C.c_uint64
         R.nbpgr_prune.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                   C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_double]
@@ -234,6 +246,30 @@ def search(rows, base, queries, dtype, L, k, cut=1.35, limit=None, degree_limit=
     st = np.zeros((nq, 2), np.uint32)
     rc = lib().nbpgo_search(C.byref(G), C.byref(B), C.byref(Q), _p(sp), 1, C.byref(qp), _p(ids),
                             _p(d), _p(st), nthreads)
+    if rc != 0:
+        raise ValueError("oracle search failed: %d" % rc)
+    return ids, d, st
+
+
+def search_exact(rows, base, queries, dtype, raw_base, raw_queries, raw_dtype, L, k,
+                 rerank_factor, cut=1.35, limit=None, degree_limit=None, entry=0, nthreads=0):
+    """Beam search over quantized rows with raw rows for the final distances
+    (rerank_factor > 0: rerank mode, 0: filtering mode); (ids, dists, stats)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    n, R1 = rows.shape
+    G = Graph(rows.ctypes.data, n, R1 - 1)
+    B, Q = points(base, dtype), points(queries, dtype)
+    XB, XQ = points(raw_base, raw_dtype), points(raw_queries, raw_dtype)
+    nq = Q.n
+    qp = QParams(k, cut, L, n if limit is None else limit,
+                 (R1 - 1) if degree_limit is None else degree_limit, 8 * L)
+    sp = np.array([entry], np.uint32)
+    ids = np.zeros((nq, k), np.uint32)
+    d = np.zeros((nq, k), np.float32)
+    st = np.zeros((nq, 2), np.uint32)
+    rc = lib().nbpgo_search_exact(C.byref(G), C.byref(B), C.byref(Q), C.byref(XB), C.byref(XQ),
+                                  _p(sp), 1, C.byref(qp), rerank_factor, _p(ids), _p(d), _p(st),
+                                  nthreads)
     if rc != 0:
         raise ValueError("oracle search failed: %d" % rc)
     return ids, d, st

@@ -337,6 +337,7 @@ typedef struct {
     nbpgo_iddist *visited;
     uint64_t visited_size, max_visited;
     nbpgo_iddist *copy_visited; /* max_visited (+R slack) */
+    float *tmp_visited;         /* filtering mode: exact distance of visited[i] */
     uint64_t num_visited;
 } cell_t;
 
@@ -354,8 +355,9 @@ static int cell_alloc(cell_t *c, uint32_t L, uint32_t R, uint64_t max_visited)
     c->candidates = (nbpgo_iddist *)malloc(sizeof(nbpgo_iddist) * ((size_t)L + R + 1));
     c->visited = (nbpgo_iddist *)malloc(sizeof(nbpgo_iddist) * (max_visited + 1));
     c->copy_visited = (nbpgo_iddist *)malloc(sizeof(nbpgo_iddist) * (max_visited + R + 1));
+    c->tmp_visited = (float *)malloc(sizeof(float) * (max_visited + 1));
     return (c->frontier && c->new_frontier && c->unvisited && c->hash && c->filtered &&
-            c->candidates && c->visited && c->copy_visited)
+            c->candidates && c->visited && c->copy_visited && c->tmp_visited)
                ? 0
                : -2;
 }
@@ -370,6 +372,7 @@ static void cell_free(cell_t *c)
     free(c->candidates);
     free(c->visited);
     free(c->copy_visited);
+    free(c->tmp_visited);
 }
 
 static void cell_clear(cell_t *c)
@@ -411,11 +414,16 @@ static inline uint32_t *g_nbrs(const nbpgo_graph *G, uint64_t v)
     return G->rows + v * (G->R + 1) + 1;
 }
 
-/* beamSearch.h:43-171, pure-quantized (track_exact_dist false).
- * `self` is the id whose row is the query (build-time search skips it,
- * beamSearch.h:79 same_as); pass 0xFFFFFFFF for external queries. */
-static uint64_t beam_search(cell_t *c, const nbpgo_graph *G, const uint8_t *q, uint32_t self,
-                            const nbpgo_points *P, const nbpgo_qparams *qp)
+/* beamSearch.h:43-171.  `self` is the id whose row is the query (build-time
+ * search skips it, beamSearch.h:79 same_as); pass 0xFFFFFFFF for external
+ * queries.  X / xq non-NULL: filtering mode (track_exact_dist): every pop
+ * that lands in the visited list records its exact distance in a parallel
+ * array moved with it (beamSearch.h:95-117); at the end the visited list
+ * takes those distances and is re-sorted (163-168).  *exact_cmps counts
+ * them. */
+static uint64_t beam_search_x(cell_t *c, const nbpgo_graph *G, const uint8_t *q, uint32_t self,
+                              const nbpgo_points *P, const nbpgo_qparams *qp,
+                              const nbpgo_points *X, const uint8_t *xq, uint64_t *exact_cmps)
 {
     const uint32_t L = qp->L;
     const uint64_t limit = qp->limit;
@@ -456,11 +464,20 @@ static uint64_t beam_search(cell_t *c, const nbpgo_graph *G, const uint8_t *q, u
         c->num_visited++;
         if (c->visited_size < c->max_visited)
             c->visited_size++;
-        if (pos < c->visited_size - 1)
+        if (pos < c->visited_size - 1) {
             memmove(c->visited + pos + 1, c->visited + pos,
                     sizeof(nbpgo_iddist) * (c->visited_size - 1 - pos));
-        if (pos <= c->visited_size - 1)
+            if (X)
+                memmove(c->tmp_visited + pos + 1, c->tmp_visited + pos,
+                        sizeof(float) * (c->visited_size - 1 - pos));
+        }
+        if (pos <= c->visited_size - 1) {
             c->visited[pos] = cur;
+            if (X) {
+                c->tmp_visited[pos] = pdist(X, cur.id, xq);
+                ++*exact_cmps;
+            }
+        }
 
         /* beamSearch.h:119-125: delayed merge */
         if (cand_n == 0 ||
@@ -534,7 +551,18 @@ static uint64_t beam_search(cell_t *c, const nbpgo_graph *G, const uint8_t *q, u
             c->unvisited[r++] = c->frontier[f++];
         remain = (long)r;
     }
+    if (X) {
+        for (uint64_t i = 0; i < c->visited_size; ++i)
+            c->visited[i].dist = c->tmp_visited[i];
+        qsort(c->visited, c->visited_size, sizeof(nbpgo_iddist), cmp_iddist);
+    }
     return cmps;
+}
+
+static uint64_t beam_search(cell_t *c, const nbpgo_graph *G, const uint8_t *q, uint32_t self,
+                            const nbpgo_points *P, const nbpgo_qparams *qp)
+{
+    return beam_search_x(c, G, q, self, P, qp, NULL, NULL, NULL);
 }
 
 int nbpgo_search(const nbpgo_graph *G, const nbpgo_points *base, const nbpgo_points *queries,
@@ -587,6 +615,88 @@ int nbpgo_search(const nbpgo_graph *G, const nbpgo_points *base, const nbpgo_poi
         }
         if (ok)
             cell_free(&c);
+    }
+    return err;
+}
+
+/* qsearchAll with full-precision rows beside the quantized ones
+ * (beam_search_rerank, beamSearch.h:183-247): rerank_factor > 0 re-scores
+ * frontier[0:min(k * rerank_factor, fs)] with the raw rows and sorts by
+ * (dist, id) (221-236); rerank_factor = 0 is the filtering mode
+ * (beam_search_x with the raw rows), returning the first
+ * min(k, fs, visited) of the exactly re-sorted visited list (217, 239-246). */
+int nbpgo_search_exact(const nbpgo_graph *G, const nbpgo_points *base,
+                       const nbpgo_points *queries, const nbpgo_points *raw_base,
+                       const nbpgo_points *raw_queries, const uint32_t *start_points,
+                       uint32_t sp_num, const nbpgo_qparams *qp, uint32_t rerank_factor,
+                       uint32_t *out_ids, float *out_dists, uint32_t *out_stats, int nthreads)
+{
+    if (qp->k > qp->L || sp_num > qp->L || raw_base->n != base->n ||
+        raw_queries->n != queries->n)
+        return -1;
+    for (uint32_t s = 0; s < sp_num; ++s)
+        if (start_points[s] >= base->n)
+            return -1;
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+    int err = 0;
+#pragma omp parallel
+    {
+        cell_t c;
+        nbpgo_iddist *pts = (nbpgo_iddist *)malloc(sizeof(nbpgo_iddist) * ((size_t)qp->L + 1));
+        int ok = cell_alloc(&c, qp->L, G->R > qp->degree_limit ? G->R : qp->degree_limit,
+                            qp->max_visited) == 0 && pts;
+        if (!ok) {
+#pragma omp atomic write
+            err = -2;
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < (int64_t)queries->n; ++i) {
+            if (!ok)
+                continue;
+            const uint8_t *q = row_of(queries, (uint64_t)i);
+            const uint8_t *xq = row_of(raw_queries, (uint64_t)i);
+            cell_clear(&c);
+            init_starting_points(&c, base, q, start_points, sp_num);
+            uint64_t exact = 0, cmps;
+            if (rerank_factor == 0)
+                cmps = beam_search_x(&c, G, q, 0xFFFFFFFFu, base, qp, raw_base, xq, &exact);
+            else
+                cmps = beam_search(&c, G, q, 0xFFFFFFFFu, base, qp);
+            cmps += exact;
+            if (c.visited_size > c.frontier_size)
+                c.visited_size = c.frontier_size;
+            const nbpgo_iddist *res;
+            uint64_t kk;
+            if (rerank_factor) {
+                uint64_t nc = (uint64_t)qp->k * rerank_factor;
+                if (nc > c.frontier_size)
+                    nc = c.frontier_size;
+                for (uint64_t j = 0; j < nc; ++j) {
+                    pts[j].id = c.frontier[j].id;
+                    pts[j].dist = pdist(raw_base, c.frontier[j].id, xq);
+                }
+                qsort(pts, nc, sizeof(nbpgo_iddist), cmp_iddist);
+                res = pts;
+                kk = nc < qp->k ? nc : qp->k;
+            } else {
+                res = c.visited;
+                kk = c.visited_size < qp->k ? c.visited_size : qp->k;
+            }
+            for (uint32_t j = 0; j < qp->k; ++j) {
+                size_t o = (size_t)i * qp->k + j;
+                out_ids[o] = j < kk ? res[j].id : 0xFFFFFFFFu;
+                if (out_dists)
+                    out_dists[o] = j < kk ? res[j].dist : INFINITY;
+            }
+            if (out_stats) {
+                out_stats[2 * i] = (uint32_t)c.num_visited;
+                out_stats[2 * i + 1] = (uint32_t)cmps;
+            }
+        }
+        if (ok)
+            cell_free(&c);
+        free(pts);
     }
     return err;
 }

@@ -82,6 +82,15 @@ int nbpgo_search(const nbpgo_graph *G, const nbpgo_points *base, const nbpgo_poi
                  const uint32_t *start_points, uint32_t sp_num, const nbpgo_qparams *qp,
                  uint32_t *out_ids, float *out_dists, uint32_t *out_stats, int nthreads);
 
+/* qsearchAll with raw rows beside the quantized ones (beamSearch.h:183-247):
+ * rerank_factor > 0 rerank mode, 0 filtering mode.  Raw rows are distances
+ * only (raw_base->n == base->n). */
+int nbpgo_search_exact(const nbpgo_graph *G, const nbpgo_points *base,
+                       const nbpgo_points *queries, const nbpgo_points *raw_base,
+                       const nbpgo_points *raw_queries, const uint32_t *start_points,
+                       uint32_t sp_num, const nbpgo_qparams *qp, uint32_t rerank_factor,
+                       uint32_t *out_ids, float *out_dists, uint32_t *out_stats, int nthreads);
+
 /* one insert-time search + prune for node `index` (nsw.h:122-175).
  * out: pruned (id, dist) list (<= R) and the beam's visited list. */
 typedef struct {

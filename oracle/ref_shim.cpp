@@ -80,6 +80,55 @@ int search_t(const uint8_t *base, uint64_t n, uint32_t dims, const uint32_t *row
     return 0;
 }
 
+// qsearchAll with the raw ranges beside the quantized ones
+// (beam_search_rerank picks rerank / filtering from rerank_factor)
+template <typename T, typename X>
+int search_exact_t(const uint8_t *base, uint64_t n, uint32_t dims, const uint32_t *rows,
+                   uint32_t R, const uint8_t *queries, uint64_t nq, const uint8_t *raw_base,
+                   uint32_t raw_dims, const uint8_t *raw_queries, uint32_t L, uint32_t k,
+                   double cut, uint64_t limit, uint32_t degree_limit, uint32_t entry,
+                   uint32_t rerank_factor, uint32_t *out_ids, uint32_t *out_stats)
+{
+    auto B = make_points<T>(base, n, dims);
+    auto Q = make_points<T>(queries, nq, dims);
+    auto XB = make_points<X>(raw_base, n, raw_dims);
+    auto XQ = make_points<X>(raw_queries, nq, raw_dims);
+    auto G = make_graph(rows, n, R);
+    QueryStatic stats(nq);
+    stats.clear();
+    uint32_t sp = entry;
+    std::function<std::pair<uint32_t *, uint32_t>(uint32_t)> get_sp =
+        [&](uint32_t) { return std::make_pair(&sp, (uint32_t)1); };
+    QueryParams QP(k, L, cut, limit, degree_limit, rerank_factor);
+    auto res = qsearchAll<uint32_t>(XQ, Q, G, XB, B, stats, get_sp, QP);
+    for (uint64_t i = 0; i < nq; ++i) {
+        for (uint32_t j = 0; j < k; ++j)
+            out_ids[i * k + j] = j < res[i].size() ? res[i][j] : 0xFFFFFFFFu;
+        out_stats[2 * i] = stats.visited[i];
+        out_stats[2 * i + 1] = stats.distances[i];
+    }
+    return 0;
+}
+
+template <typename T>
+int search_exact_x(int raw_dtype, const uint8_t *base, uint64_t n, uint32_t dims,
+                   const uint32_t *rows, uint32_t R, const uint8_t *queries, uint64_t nq,
+                   const uint8_t *raw_base, uint32_t raw_dims, const uint8_t *raw_queries,
+                   uint32_t L, uint32_t k, double cut, uint64_t limit, uint32_t degree_limit,
+                   uint32_t entry, uint32_t rerank_factor, uint32_t *out_ids, uint32_t *out_stats)
+{
+#define NBPGR_X(XT)                                                                                \
+    search_exact_t<T, XT>(base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, \
+                          k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats)
+    switch (raw_dtype) {
+    case 0: return NBPGR_X(uint8_t);
+    case 1: return NBPGR_X(int8_t);
+    case 4: return NBPGR_X(float);
+    }
+#undef NBPGR_X
+    return -1;
+}
+
 template <typename T>
 uint64_t prune_t(const uint8_t *base, uint64_t n, uint32_t dims, uint32_t p, IdDist<uint32_t> *cand,
                  uint64_t size, uint32_t R, double alpha, double cos_angle)
@@ -151,6 +200,20 @@ int nbpgr_search(int dtype, const uint8_t *base, uint64_t n, uint32_t dims, cons
     case 1: return search_t<int8_t>(base, n, dims, rows, R, queries, nq, L, k, cut, limit, degree_limit, entry, out_ids, out_stats);
     case 2: return search_t<uint4_2>(base, n, dims, rows, R, queries, nq, L, k, cut, limit, degree_limit, entry, out_ids, out_stats);
     case 3: return search_t<int4_2>(base, n, dims, rows, R, queries, nq, L, k, cut, limit, degree_limit, entry, out_ids, out_stats);
+    }
+    return -1;
+}
+
+int nbpgr_search_exact(int dtype, const uint8_t *base, uint64_t n, uint32_t dims,
+                       const uint32_t *rows, uint32_t R, const uint8_t *queries, uint64_t nq,
+                       int raw_dtype, const uint8_t *raw_base, uint32_t raw_dims,
+                       const uint8_t *raw_queries, uint32_t L, uint32_t k, double cut,
+                       uint64_t limit, uint32_t degree_limit, uint32_t entry,
+                       uint32_t rerank_factor, uint32_t *out_ids, uint32_t *out_stats)
+{
+    switch (dtype) {
+    case 0: return search_exact_x<uint8_t>(raw_dtype, base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats);
+    case 2: return search_exact_x<uint4_2>(raw_dtype, base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats);
     }
     return -1;
 }

@@ -66,3 +66,62 @@ def test_rerank_matches_oracle(gpu, raw_kind):
         N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(),
                  torch.zeros((len(rq), 300), dtype=torch.int32, device="cuda"), 100, 3)
     ix.close()
+
+
+WORKLOADS = [("f32", "U8", "F32"), ("u8", "U4X2", "U8")]
+
+
+def _exact_index(N, z, name, cdt):
+    import torch
+    codes = z[name + "_codes"]
+    ix = N.Index(getattr(N, cdt), codes.shape[0], codes.shape[1], 16)
+    ix.set_points(codes)
+    ix.set_graph(z[name + "_graph"])
+    qc = z[name + "_qcodes"]
+    qpad = np.zeros((len(qc), ix.row_stride), np.uint8)
+    qpad[:, :qc.shape[1]] = qc
+    return (ix, N.to_device(qpad), torch.from_numpy(z[name + "_raw"]).cuda(),
+            torch.from_numpy(z[name + "_rawq"]).cuda())
+
+
+@pytest.mark.parametrize("name,cdt,xdt", WORKLOADS)
+def test_search_rerank_ref_matches_reference(gpu, golden_exact, name, cdt, xdt):
+    """Reference mode with raw rows: rerank (rerank_factor > 0) and filtering
+    (0) return the reference's ids and {visited, dist_cmps}, with the raw
+    rows' distances."""
+    N, z = gpu, golden_exact
+    ix, qd, xb, xq = _exact_index(N, z, name, cdt)
+    for i, (L, k, rf) in enumerate(z["settings"]):
+        L, k, rf = int(L), int(k), int(rf)
+        ids, d, st = N.search_rerank(ix, qd, xb, xq, L, k, rf, mode=N.SEARCH_REF)
+        assert (ids.cpu().numpy().view(np.uint32) == z[name + "_ids"][i][:, :k]).all(), (L, k, rf)
+        assert (st.cpu().numpy().view(np.uint32) == z[name + "_stats"][i]).all(), (L, k, rf)
+        _, od, _ = O.search_exact(z[name + "_graph"], z[name + "_codes"], z[name + "_qcodes"],
+                                  getattr(O, cdt), z[name + "_raw"], z[name + "_rawq"],
+                                  getattr(O, xdt), L, k, rf)
+        assert (d.cpu().numpy() == od).all(), (L, k, rf)
+    ix.close()
+
+
+@pytest.mark.parametrize("name,cdt,xdt", WORKLOADS)
+def test_frontier_output_and_fast_rerank(gpu, golden_exact, name, cdt, xdt):
+    """The whole-frontier output feeds the stand-alone rerank to the fused
+    reference-mode result; the FAST form equals its beam re-scored."""
+    N, z = gpu, golden_exact
+    ix, qd, xb, xq = _exact_index(N, z, name, cdt)
+    for L, k, rf in [(20, 10, 1), (20, 10, 3), (40, 10, 100), (32, 5, 2)]:
+        fr, _, _ = N.search(ix, qd, L, k, mode=N.SEARCH_REF, frontier=True)
+        head, _, _ = N.search(ix, qd, L, k, mode=N.SEARCH_REF)
+        assert fr.shape == (len(qd), L)
+        assert (fr[:, :k] == head).all()
+        a, ad = N.rerank(xb, xq, fr, k, rf)
+        b, bd, _ = N.search_rerank(ix, qd, xb, xq, L, k, rf, mode=N.SEARCH_REF)
+        assert (a == b).all() and (ad == bd).all(), (L, k, rf)
+        beam, _, _ = N.search(ix, qd, L, L, mode=N.SEARCH_FAST)
+        c, cd = N.rerank(xb, xq, beam, k, rf)
+        f, fd, fst = N.search_rerank(ix, qd, xb, xq, L, k, rf, mode=N.SEARCH_FAST)
+        assert (c == f).all() and (cd == fd).all(), (L, k, rf)
+        assert (fst[:, 0] > 0).all()
+    with pytest.raises(N.NbpgError):
+        N.search_rerank(ix, qd, xb, xq, 20, 10, 0, mode=N.SEARCH_FAST)
+    ix.close()

@@ -233,3 +233,23 @@ def test_matmul_against_float64():
     for i in range(96):
         s = np.float32(s + np.float32(W[0, i] * x[0, i]))
     assert y[0, 0] == np.float32(b[0] + s)
+
+
+@pytest.mark.parametrize("name,cdt,xdt", [("f32", O.U8, O.F32), ("u8", O.U4X2, O.U8)])
+def test_search_exact_golden(golden_exact, name, cdt, xdt):
+    """Rerank (rerank_factor > 0) and filtering (0) searches over quantized
+    rows equal the reference's qsearchAll: ids and {visited, dist_cmps}; the
+    distances are the raw rows' own (closed form)."""
+    z = golden_exact
+    xb, xq = z[name + "_raw"], z[name + "_rawq"]
+    for i, (L, k, rf) in enumerate(z["settings"]):
+        ids, d, st = O.search_exact(z[name + "_graph"], z[name + "_codes"], z[name + "_qcodes"],
+                                    cdt, xb, xq, xdt, int(L), int(k), int(rf))
+        assert (ids == z[name + "_ids"][i][:, :k]).all(), (L, k, rf)
+        assert (st == z[name + "_stats"][i]).all(), (L, k, rf)
+        for q in range(0, len(xq), 7):
+            for j in range(k):
+                if ids[q, j] != 0xFFFFFFFF:
+                    assert d[q, j] == O.distance(xdt, xb[ids[q, j]], xq[q])
+            live = ids[q] != 0xFFFFFFFF
+            assert (np.diff(d[q][live]) >= 0).all()
