@@ -159,7 +159,7 @@ int nbpg_search(const nbpg_index *idx, const void *d_queries, uint64_t nq, uint6
  *     dist_cmps counter adds those exact distances as the reference does.
  * NBPG_SEARCH_REF returns the reference's IDs (f32 rows: distances summed in
  * index order, point.h:137-146).  NBPG_SEARCH_FAST re-scores the top
- * min(k * rerank_factor, L) <= 256 of its beam, using the index's workspace
+ * min(k * rerank_factor, L) <= 1024 of its beam, using the index's workspace
  * (calls on one index must not overlap).  Outputs nq x k. */
 int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq,
                        uint64_t query_stride, int raw_dtype, const void *d_raw_base,
@@ -198,7 +198,7 @@ int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries
 /* ---- rerank: the second half of beam_search_rerank (beamSearch.h:183-236).
  * d_cand holds each query's frontier from a search over quantized codes
  * (nq rows of cand_stride ids, first n_cand used, an id >= n ends a row).
- * The first min(k * rerank_factor, n_cand) candidates (at most 256) are
+ * The first min(k * rerank_factor, n_cand) candidates (at most 1024) are
  * re-scored against the full-precision rows (raw_dtype NBPG_F32 / NBPG_U8 /
  * NBPG_I8, dims elements each, packed) and the k nearest by (dist, id) are
  * written, nq x k; missing slots get NBPG_SENT / +inf.  f32 distances are the

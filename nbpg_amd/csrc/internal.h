@@ -63,7 +63,7 @@ extern DevInfo g_dev;
 int ensure_init();
 int ensure_ws(nbpg_index *idx, size_t bytes);
 // rerank of candidate rows (rerank.hip); raw query rows at q_stride bytes
-#define RERANK_MAX 256
+#define RERANK_MAX 1024
 int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
                   const void *d_raw_queries, uint64_t q_stride, uint64_t nq,
                   const uint32_t *d_cand, uint32_t cand_stride, uint32_t n_cand, uint32_t k,

@@ -6,7 +6,7 @@
 // One wavefront per query.  Each lane scores whole candidates: it streams its
 // candidate's raw row in index order, so a float distance is the reference's
 // sequential fp32 sum (point.h:137-146) bit for bit and an integer one is
-// exact.  The <= 256 keys are then sorted in LDS.
+// exact.  The <= 1024 keys are then sorted in LDS.
 #include <algorithm>
 
 #include "raw_rows.h"

@@ -70,7 +70,7 @@ def build_tools():
     out = []
     for name, src in TOOLS.items():
         exe = os.path.join(BIN, name)
-        cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+        cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
                "-I" + os.path.join(HERE, "..", "include"), "-I" + os.path.join(ROCM, "include"),
                os.path.join(HERE, "tools", src), "-o", exe + ".tmp",
                "-L" + HERE, "-lnbpg_amd", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",

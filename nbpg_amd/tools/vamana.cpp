@@ -11,11 +11,19 @@
 //           [--q_num 0] [--verbose]]
 //          [--search_mode ref|fast] [--sp_seed 1]
 //
+//          [--base_path X.{fbin,bin,u8bin,i8bin,u4bin,i4bin} --query_path XQ
+//           [--rerank_factor 100] [--cache_raw]]
+//
 // --search_mode ref (the default) returns the reference's IDs and counters;
-// fast runs the persistent kernel.  4-split builds draw their random start
-// points from --sp_seed instead of std::random_device.  Not on this path
-// (refused): --use_mips, --dir_bias_scale > 0, rerank over --base_path /
-// --query_path rows that differ from the quantized ones.
+// fast runs the persistent kernel.  With --base_path / --query_path (raw
+// rows of another size than the quantized ones) the sweep searches the
+// quantized rows and takes final distances from the raw ones
+// (beam_search_rerank, beamSearch.h:183-247): --rerank_factor > 0 re-scores
+// the frontier head, 0 is the filtering mode (reference mode only); the raw
+// rows stay in device memory (--cache_raw is accepted and changes nothing).
+// 4-split builds draw their random start points from --sp_seed instead of
+// std::random_device.  Not on this path (refused): --use_mips,
+// --dir_bias_scale > 0.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -40,6 +48,15 @@ bool file_exists(const std::string &p)
 // point.h:35-120 on the host, for the tie extension of the recall
 float host_dist(int dtype, const uint8_t *a, const uint8_t *b, uint32_t dims)
 {
+    if (dtype == NBPG_F32) { // point.h:137-146, summed in index order
+        const float *p = reinterpret_cast<const float *>(a), *q = reinterpret_cast<const float *>(b);
+        float r = 0.0f;
+        for (uint32_t i = 0; i < dims; ++i) {
+            const float d = p[i] - q[i];
+            r += d * d;
+        }
+        return r;
+    }
     int64_t s = 0;
     for (uint32_t i = 0; i < dims; ++i) {
         if (dtype == NBPG_U8) {
@@ -205,9 +222,6 @@ int main(int argc, char **argv)
             nbpg_index_destroy(ix);
             return 0;
         }
-        if ((!f.str("base_path").empty() && f.str("base_path") != bp) ||
-            (!f.str("query_path").empty() && f.str("query_path") != qp))
-            throw std::runtime_error("rerank over separate raw rows is not on this path");
         Rows q = read_rows(qp);
         const uint64_t qn = f.num("q_num", 0) > 0 ? std::min<uint64_t>(q.n, (uint64_t)f.num("q_num", 0)) : q.n;
         uint32_t gnq = 0, gk = 0;
@@ -225,36 +239,86 @@ int main(int argc, char **argv)
         const int mode = sm == "ref" ? NBPG_SEARCH_REF : NBPG_SEARCH_FAST;
         DevBuf dq(q.data.data(), qn * q.row_bytes());
 
-        auto check = [&](uint32_t k, uint32_t Q, double cut, uint64_t limit, uint32_t dlim) -> Result {
-            DevBuf di(qn * k * 4), dd(qn * k * 4), ds(qn * 8);
-            nbpg_query_params qpm = {k, cut, Q, limit, dlim};
+        // raw rows (Vamana.cpp:212-272): recall always measures against them;
+        // the search uses them when their row size differs from the
+        // quantized rows' (beamSearch.h:21-38)
+        const std::string xbp = f.str("base_path"), xqp = f.str("query_path");
+        if (xbp.empty() != xqp.empty())
+            throw std::runtime_error("--base_path and --query_path go together");
+        Rows xb, xq;
+        int xdtype = dtype;
+        if (!xbp.empty()) {
+            if (ends_with(xbp, ".u8bin"))
+                xdtype = NBPG_U8;
+            else if (ends_with(xbp, ".i8bin"))
+                xdtype = NBPG_I8;
+            else if (ends_with(xbp, ".u4bin"))
+                xdtype = NBPG_U4X2;
+            else if (ends_with(xbp, ".i4bin"))
+                xdtype = NBPG_I4X2;
+            else if (ends_with(xbp, ".fbin") || ends_with(xbp, ".bin"))
+                xdtype = NBPG_F32;
+            else
+                throw std::runtime_error("--base_path must be .u8bin, .i8bin, .u4bin, .i4bin, .fbin or .bin");
+            xb = read_rows(xbp);
+            xq = read_rows(xqp, qn);
+            if (xb.n != n || xq.n < qn || xq.dims != xb.dims || xq.elem != xb.elem)
+                throw std::runtime_error("raw rows do not match the quantized rows");
+        }
+        const Rows &rb = xbp.empty() ? base : xb;
+        const Rows &rq = xbp.empty() ? q : xq;
+        const bool exact = !xbp.empty() && xb.row_bytes() != base.row_bytes();
+        DevBuf dxb(exact ? xb.data.size() : 0), dxq(exact ? qn * xq.row_bytes() : 0);
+        if (exact) {
+            hip_ok(hipMemcpy(dxb.p, xb.data.data(), dxb.bytes, hipMemcpyHostToDevice), "upload");
+            hip_ok(hipMemcpy(dxq.p, xq.data.data(), dxq.bytes, hipMemcpyHostToDevice), "upload");
+        }
+        const uint32_t flag_rf = (uint32_t)f.num("rerank_factor", 100);
+
+        // checkRecall (check_nn_recall.h:51-80): search with qk results,
+        // recall over the first k
+        auto check = [&](uint32_t k, uint32_t qk, uint32_t Q, double cut, uint64_t limit,
+                         uint32_t dlim, uint32_t rf) -> Result {
+            DevBuf di(qn * qk * 4), dd(qn * qk * 4), ds(qn * 8);
+            nbpg_query_params qpm = {qk, cut, Q, limit, dlim};
             const uint32_t sp = 0;
             const auto t0 = std::chrono::steady_clock::now();
-            nbpg_ok(nbpg_search(ix, dq.p, qn, q.row_bytes(), &sp, 1, &qpm, mode, nullptr,
-                                (uint32_t *)di.p, (float *)dd.p, (uint32_t *)ds.p, nullptr),
-                    "nbpg_search");
+            if (exact)
+                nbpg_ok(nbpg_search_rerank(ix, dq.p, qn, q.row_bytes(), xdtype, dxb.p, xb.dims,
+                                           dxq.p, xq.row_bytes(), &sp, 1, &qpm, rf, mode, nullptr,
+                                           (uint32_t *)di.p, (float *)dd.p, (uint32_t *)ds.p,
+                                           nullptr),
+                        "nbpg_search_rerank");
+            else
+                nbpg_ok(nbpg_search(ix, dq.p, qn, q.row_bytes(), &sp, 1, &qpm, mode, nullptr,
+                                    (uint32_t *)di.p, (float *)dd.p, (uint32_t *)ds.p, nullptr),
+                        "nbpg_search");
             hip_ok(hipDeviceSynchronize(), "search");
             const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            std::vector<uint32_t> ids(qn * k), st(qn * 2);
+            std::vector<uint32_t> ids(qn * qk), st(qn * 2);
             di.download(ids.data(), ids.size() * 4);
             ds.download(st.data(), st.size() * 4);
             // recall with ties (check_nn_recall.h:126-167)
             uint64_t correct = 0;
             for (uint64_t i = 0; i < qn; ++i) {
-                const uint8_t *qrow = q.data.data() + i * q.row_bytes();
+                const uint8_t *qrow = rq.data.data() + i * rq.row_bytes();
+                auto dist = [&](uint32_t id) {
+                    return host_dist(xdtype, rb.data.data() + (uint64_t)id * rb.row_bytes(), qrow,
+                                     rb.dims);
+                };
                 std::vector<uint32_t> with_ties(gt.begin() + i * gk, gt.begin() + i * gk + k);
-                const float last = host_dist(dtype, base.data.data() + (uint64_t)gt[i * gk + k - 1] * base.row_bytes(), qrow, base.dims);
+                const float last = dist(gt[i * gk + k - 1]);
                 for (uint32_t l = k; l < gk; ++l) {
                     const uint32_t g = gt[i * gk + l];
-                    if (host_dist(dtype, base.data.data() + (uint64_t)g * base.row_bytes(), qrow, base.dims) == last)
+                    if (dist(g) == last)
                         with_ties.push_back(g);
                     else
                         break;
                 }
                 std::set<uint32_t> rep;
-                for (uint32_t l = 0; l < k; ++l)
-                    if (ids[i * k + l] != 0xFFFFFFFFu)
-                        rep.insert(ids[i * k + l]);
+                for (uint32_t l = 0; l < k && l < qk; ++l)
+                    if (ids[i * qk + l] != 0xFFFFFFFFu)
+                        rep.insert(ids[i * qk + l]);
                 for (uint32_t g : with_ties)
                     correct += rep.count(g);
             }
@@ -264,11 +328,13 @@ int main(int argc, char **argv)
                 cmp[i] = st[2 * i + 1];
             }
             auto dsx = statistics(cmp), vsx = statistics(vis);
-            Result r{(double)correct / (double)(k * qn), dsx.first, dsx.second, vsx.first, vsx.second,
+            // a float ratio, as check_nn_recall.h:165 computes it
+            const float recall = static_cast<float>(correct) / static_cast<float>(k * qn);
+            Result r{recall, dsx.first, dsx.second, vsx.first, vsx.second,
                      (float)(qn / t), k, Q, (float)cut, limit, dlim, qn};
             if (verbose)
-                std::cout << "search: Q=" << Q << ", k=" << k << ", limit=" << limit << ", recall="
-                          << r.recall << ", visited=" << r.avg_visited << ", comparisons="
+                std::cout << "search: Q=" << Q << ", k=" << qk << ", limit=" << limit << ", recall="
+                          << recall << ", visited=" << r.avg_visited << ", comparisons="
                           << r.avg_cmps << ", QPS=" << r.qps << std::endl;
             return r;
         };
@@ -276,7 +342,7 @@ int main(int argc, char **argv)
         const uint32_t fixedQ = (uint32_t)f.num("Q", 0);
         if (fixedQ) {
             for (int i = 0; i < 5; ++i)
-                check(kk, fixedQ, 1.35, n, gR).print();
+                check(kk, kk, fixedQ, 1.35, n, gR, flag_rf).print();
             nbpg_index_destroy(ix);
             return 0;
         }
@@ -287,11 +353,12 @@ int main(int argc, char **argv)
                                   120, 140, 160, 180, 200, 225, 250, 275, 300, 375, 500, 750, 1000};
         for (uint32_t Q : beams)
             if (Q >= kk)
-                results.push_back(check(kk, Q, 1.35, n, gR));
+                results.push_back(check(kk, kk, Q, 1.35, n, gR, flag_rf));
+        // the limited and best-accuracy points rebuild QueryParams with its
+        // default rerank factor 100 (check_nn_recall.h:236-248, query_params.h:34-35)
         for (uint32_t l : {10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 28, 30, 35})
-            results.push_back(check(kk, std::max(l, kk), 1.35, l, std::min(gR, 5 * l)));
-        if (100 <= gk)
-            results.push_back(check(100, 1000, 10.0, n, gR));
+            results.push_back(check(kk, kk, std::max(l, kk), 1.35, l, std::min(gR, 5 * l), 100));
+        results.push_back(check(kk, 100, 1000, 10.0, n, gR, 100));
         const float buckets[] = {.1f, .2f, .3f, .4f, .5f, .6f, .7f, .75f, .8f, .85f, .9f, .93f, .95f,
                                  .97f, .98f, .99f, .995f, .999f, .9995f, .9999f, .99995f, .99999f};
         const int nb = sizeof(buckets) / sizeof(buckets[0]);

@@ -49,7 +49,7 @@ def test_rerank_matches_oracle(gpu, raw_kind):
     qpad[:, :qcodes.shape[1]] = qcodes
     L = 40
     cand, _, _ = N.search(ix, N.to_device(qpad), L, L, mode=N.SEARCH_FAST)
-    for k, factor in [(10, 3), (10, 100), (1, 5), (32, 1)]:
+    for k, factor in [(10, 3), (10, 100), (1, 5), (32, 1), (40, 1)]:
         ids, d = N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(), cand, k,
                           factor)
         ei, ed = expected(raw, rq, cand.cpu().numpy().view(np.uint32), k, factor, rdt)
@@ -64,7 +64,18 @@ def test_rerank_matches_oracle(gpu, raw_kind):
         assert r_re >= r_q
     with pytest.raises(N.NbpgError):
         N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(),
-                 torch.zeros((len(rq), 300), dtype=torch.int32, device="cuda"), 100, 3)
+                 torch.zeros((len(rq), 1100), dtype=torch.int32, device="cuda"), 100, 11)
+    # long candidate lists (up to the 1024 limit), some ended early
+    rng = np.random.default_rng(5)
+    cand = np.stack([rng.permutation(len(raw))[:1024] for _ in range(len(rq))]).astype(np.uint32)
+    cand[3, 500:] = 0xFFFFFFFF
+    cand[7, 0:] = 0xFFFFFFFF
+    cd = torch.from_numpy(cand.view(np.int32)).cuda()
+    for k, factor in [(50, 14), (100, 11), (7, 1000)]:
+        ids, d = N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(), cd, k, factor)
+        ei, ed = expected(raw, rq, cand, k, factor, rdt)
+        assert (ids.cpu().numpy().view(np.uint32) == ei).all(), (k, factor)
+        assert (d.cpu().numpy() == ed).all(), (k, factor)
     ix.close()
 
 

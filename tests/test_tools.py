@@ -105,6 +105,58 @@ def test_vamana_tool_builds_the_reference_graph(gpu, golden, tmp_path):
     assert rows and all(int(x[0]) == 40 for x in rows)
 
 
+def write_gt(path, ids, d):
+    with open(path, "wb") as f:
+        np.array(ids.shape, np.uint32).tofile(f)
+        np.ascontiguousarray(ids, np.uint32).tofile(f)
+        np.ascontiguousarray(d, np.float32).tofile(f)
+
+
+def sweep_lines(out):
+    """the per-point search lines of a --verbose sweep, without the timing"""
+    return [ln.split(", QPS=")[0] for ln in out.splitlines() if ln.startswith("search: ")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ext,xext,rf", [("f32", "u8bin", "fbin", "100"),
+                                              ("f32", "u8bin", "fbin", "0"),
+                                              ("u8", "u4bin", "u8bin", "2")])
+def test_vamana_tool_rerank_sweep_matches_reference_binary(gpu, golden_exact, tmp_path, name,
+                                                           ext, xext, rf):
+    """Quantized graph + raw rows: every point of the recall sweep (recall,
+    mean visited, mean comparisons) equals the reference binary's."""
+    import oracle as O
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "Vamana")
+    if not os.path.exists(ref_bin):
+        pytest.skip("oracle/_ref not built")
+    z = golden_exact
+    write_bin(tmp_path / ("b." + ext), z[name + "_codes"])
+    write_bin(tmp_path / ("q." + ext), z[name + "_qcodes"])
+    write_bin(tmp_path / ("x." + xext), z[name + "_raw"])
+    write_bin(tmp_path / ("xq." + xext), z[name + "_rawq"])
+    g = z[name + "_graph"]
+    with open(tmp_path / "g.graph", "wb") as f:
+        np.array([g.shape[0], g.shape[1] - 1], np.uint32).tofile(f)
+        g[:, 0].tofile(f)
+        for row in g:
+            row[1:1 + row[0]].tofile(f)
+    xdt = O.F32 if xext == "fbin" else O.U8
+    gi, gd = O.compgt(z[name + "_raw"], z[name + "_rawq"], 100, xdt, tie_mode=1)
+    write_gt(tmp_path / "g.gt", gi, gd)
+    args = ["--quant_base_path", str(tmp_path / ("b." + ext)), "--index_path", str(tmp_path / "g"),
+            "--quant_query_path", str(tmp_path / ("q." + ext)), "--base_path",
+            str(tmp_path / ("x." + xext)), "--query_path", str(tmp_path / ("xq." + xext)),
+            "--gt_path", str(tmp_path / "g.gt"), "--k", "10", "--rerank_factor", rf, "--verbose"]
+    ours = run([tool("Vamana")] + args + ["--res_path", str(tmp_path / "ours.csv")])
+    assert ours.returncode == 0, ours.stderr
+    ref = subprocess.run([ref_bin] + args + ["--res_path", str(tmp_path / "ref.csv")],
+                         capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    a, b = sweep_lines(ours.stdout), sweep_lines(ref.stdout)
+    assert len(a) == len(b) > 60
+    assert a == b
+
+
 @pytest.mark.gpu
 def test_pcacompress_tool_writes_the_reference_codes(gpu, tmp_path):
     g = np.load(os.path.join(GOLD, "reference_pca.npz"))
