@@ -359,11 +359,14 @@ def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100, raw_dtype=No
     torch = _torch()
     nq, C = cand_dev.shape
     n, dims = raw_base_dev.shape
+    if cand_dev.stride(1) != 1 or not raw_base_dev.is_contiguous() or not raw_q_dev.is_contiguous():
+        raise NbpgError("rerank needs packed raw rows and unit-stride candidate rows")
     ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
     d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
     rdt = _x_dtype(raw_base_dev) if raw_dtype is None else raw_dtype
     _check(lib().nbpg_rerank(rdt, _ptr(raw_base_dev), n, dims, _ptr(raw_q_dev),
-                             nq, _ptr(cand_dev), C, C, k, rerank_factor, _ptr(ids), _ptr(d),
+                             nq, _ptr(cand_dev), cand_dev.stride(0), C, k, rerank_factor,
+                             _ptr(ids), _ptr(d),
                              _stream()), "nbpg_rerank")
     return ids, d
 

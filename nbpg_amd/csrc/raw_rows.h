@@ -76,6 +76,56 @@ template <> struct raw_row<NBPG_U4X2> {
     }
 };
 
+// The same distances accumulated over consecutive pieces of a row (nbytes
+// each, in index order), so a row can be streamed through LDS in chunks:
+// acc = raw_acc<RT>::add(acc, x, y, nbytes) ... raw_acc<RT>::fin(acc).
+template <int RT> struct raw_acc;
+template <> struct raw_acc<NBPG_F32> {
+    typedef float T;
+    static __device__ __forceinline__ T add(T s, const uint8_t *a, const uint8_t *b, uint32_t nb)
+    {
+        const float4 *p = reinterpret_cast<const float4 *>(a), *q = reinterpret_cast<const float4 *>(b);
+        for (uint32_t i = 0; i < nb / 16; ++i) {
+            const float4 x = p[i], y = q[i];
+            float d = __fsub_rn(x.x, y.x);
+            s = __fadd_rn(s, __fmul_rn(d, d));
+            d = __fsub_rn(x.y, y.y);
+            s = __fadd_rn(s, __fmul_rn(d, d));
+            d = __fsub_rn(x.z, y.z);
+            s = __fadd_rn(s, __fmul_rn(d, d));
+            d = __fsub_rn(x.w, y.w);
+            s = __fadd_rn(s, __fmul_rn(d, d));
+        }
+        return s;
+    }
+    static __device__ __forceinline__ float fin(T s) { return s; }
+};
+template <int RT> struct raw_acc_int {
+    typedef int32_t T;
+    static __device__ __forceinline__ T add(T s, const uint8_t *a, const uint8_t *b, uint32_t nb)
+    {
+        for (uint32_t i = 0; i < nb; ++i) {
+            if (RT == NBPG_U8) {
+                const int32_t d = (int32_t)a[i] - (int32_t)b[i];
+                s += d * d;
+            } else if (RT == NBPG_I8) {
+                const int32_t d = (int32_t)(int8_t)a[i] - (int32_t)(int8_t)b[i];
+                s += d * d;
+            } else {
+                const int32_t dh = (int32_t)(a[i] & 0xF0) - (int32_t)(b[i] & 0xF0);
+                const int32_t dl = (int32_t)((a[i] << 4) & 0xF0) - (int32_t)((b[i] << 4) & 0xF0);
+                s += dh * dh;
+                s += dl * dl;
+            }
+        }
+        return s;
+    }
+    static __device__ __forceinline__ float fin(T s) { return (float)s; }
+};
+template <> struct raw_acc<NBPG_U8> : raw_acc_int<NBPG_U8> {};
+template <> struct raw_acc<NBPG_I8> : raw_acc_int<NBPG_I8> {};
+template <> struct raw_acc<NBPG_U4X2> : raw_acc_int<NBPG_U4X2> {};
+
 // the raw dtype chosen at run time (uniform across the wavefront)
 __device__ __forceinline__ float raw_dist(int rt, const uint8_t *a, const uint8_t *b, uint32_t dims)
 {

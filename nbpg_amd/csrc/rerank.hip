@@ -3,10 +3,12 @@
 // num_check = min(k * rerank_factor, frontier size) entries are re-scored with
 // the full-precision rows, sorted by (dist, id) and cut to k.
 //
-// One wavefront per query.  Each lane scores whole candidates: it streams its
+// One wavefront per query.  Each lane scores whole candidates, summing its
 // candidate's raw row in index order, so a float distance is the reference's
 // sequential fp32 sum (point.h:137-146) bit for bit and an integer one is
-// exact.  The <= 1024 keys are then sorted in LDS.
+// exact.  Aligned rows are staged through LDS with whole-line loads (the
+// staged kernel); others are read lane by lane.  The <= 1024 keys are then
+// sorted in LDS.
 #include <algorithm>
 
 #include "raw_rows.h"
@@ -22,17 +24,9 @@ struct RerankArgs {
     float *dists;
 };
 
-template <int RT>
-__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
+// frontier length of query q: its first id >= n ends it (<= num_check)
+__device__ __forceinline__ uint32_t frontier_len(const RerankArgs &a, uint64_t q, uint32_t lane)
 {
-    __shared__ uint64_t keys[4][RERANK_MAX];
-    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-    const uint64_t q = (uint64_t)blockIdx.x * 4 + wv;
-    if (q >= a.nq)
-        return;
-    uint64_t *kw = keys[wv];
-    const uint8_t *qrow = a.queries + q * a.qstride;
-    // the frontier ends at its first empty slot
     uint32_t m = a.num_check;
     for (uint32_t i = lane; i < a.num_check; i += WAVE) {
         const uint32_t id = a.cand[q * a.cand_stride + i];
@@ -42,17 +36,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         m = min(m, (uint32_t)__shfl_xor(m, o));
-    uint32_t n2 = 1;
-    while (n2 < m)
-        n2 <<= 1;
-    for (uint32_t i = lane; i < n2; i += WAVE) {
-        uint64_t key = ~0ull;
-        if (i < m) {
-            const uint32_t id = a.cand[q * a.cand_stride + i];
-            key = make_key(raw_row<RT>::dist(a.base + (uint64_t)id * a.row_bytes, qrow, a.dims), id);
-        }
-        kw[i] = key;
-    }
+    return m;
+}
+
+// kw[0:n2) holds m keys then ~0: bitonic sort, then the first k out
+__device__ __forceinline__ void sort_and_write(const RerankArgs &a, uint64_t q, uint64_t *kw,
+                                               uint32_t m, uint32_t n2, uint32_t lane)
+{
     wave_sync();
     for (uint32_t kk = 2; kk <= n2; kk <<= 1)
         for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
@@ -74,6 +64,111 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
         if (a.dists)
             a.dists[q * a.k + i] = ok ? key_dist(kw[i]) : __int_as_float(0x7f800000);
     }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t m)
+{
+    uint32_t n2 = 1;
+    while (n2 < m)
+        n2 <<= 1;
+    return n2;
+}
+
+// any row size or alignment: each lane streams whole rows from memory
+template <int RT>
+__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
+{
+    __shared__ uint64_t keys[4][RERANK_MAX];
+    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const uint64_t q = (uint64_t)blockIdx.x * 4 + wv;
+    if (q >= a.nq)
+        return;
+    uint64_t *kw = keys[wv];
+    const uint8_t *qrow = a.queries + q * a.qstride;
+    const uint32_t m = frontier_len(a, q, lane), n2 = pow2_at_least(m);
+    for (uint32_t i = lane; i < n2; i += WAVE) {
+        uint64_t key = ~0ull;
+        if (i < m) {
+            const uint32_t id = a.cand[q * a.cand_stride + i];
+            key = make_key(raw_row<RT>::dist(a.base + (uint64_t)id * a.row_bytes, qrow, a.dims), id);
+        }
+        kw[i] = key;
+    }
+    sort_and_write(a, q, kw, m, n2, lane);
+}
+
+// 16-byte aligned rows (row bytes % 16 == 0): 64 candidates at a time, their
+// rows staged through LDS 128 bytes per row per step with whole-line loads
+// (one load instruction covers 8 rows x 128 B), then each lane sums its own
+// row's piece from LDS in index order -- the same sequential sum as above.
+#define RS_CH 128                      // bytes of each row per step
+#define RS_PITCH 144                   // LDS row pitch (16-byte pad against bank conflicts)
+#define RS_STAGE (65 * RS_PITCH)       // 64 candidate rows + the query's piece
+
+typedef uint32_t rs_vec __attribute__((ext_vector_type(4))); // a 16-byte piece
+
+// one step's loads: lane (sub, piece) takes piece `piece` (16 B) of rows
+// t * 8 + sub, t = 0..7, and lanes 0..7 the query's pieces
+__device__ __forceinline__ void rs_fetch(rs_vec (&buf)[8], rs_vec &qbuf, const uint8_t *base,
+                                         uint64_t row_bytes, uint32_t my_id, const uint8_t *qrow,
+                                         uint64_t c0, uint32_t gm, uint32_t piece, uint32_t sub)
+{
+    const uint64_t off = c0 + 16u * piece;
+    const bool in_row = off < row_bytes;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const uint32_t id = (uint32_t)__shfl((int)my_id, t * 8 + (int)sub);
+        if ((uint32_t)t * 8 + sub < gm && in_row)
+            buf[t] = *reinterpret_cast<const rs_vec *>(base + (uint64_t)id * row_bytes + off);
+    }
+    if (sub == 0 && in_row)
+        qbuf = *reinterpret_cast<const rs_vec *>(qrow + off);
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32_t n2cap)
+{
+    extern __shared__ __align__(16) uint8_t lds_rr[];
+    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wv;
+    if (q >= a.nq)
+        return;
+    uint8_t *stage = lds_rr + (uint64_t)wv * (RS_STAGE + 8ull * n2cap);
+    uint64_t *kw = reinterpret_cast<uint64_t *>(stage + RS_STAGE);
+    const uint8_t *qrow = a.queries + q * a.qstride;
+    const uint32_t *cand = a.cand + q * a.cand_stride;
+    const uint32_t m = frontier_len(a, q, lane), n2 = pow2_at_least(m);
+    const uint32_t piece = lane % 8, sub = lane / 8;
+    for (uint32_t g0 = 0; g0 < m; g0 += WAVE) {
+        const uint32_t gm = min((uint32_t)WAVE, m - g0);
+        const uint32_t my_id = lane < gm ? cand[g0 + lane] : 0u;
+        // the next step's pieces are loaded into registers while this
+        // step's are summed (two steps in flight per wave)
+        rs_vec buf[8], qbuf = {0, 0, 0, 0};
+        typename raw_acc<RT>::T acc = 0;
+        rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, 0, gm, piece, sub);
+        for (uint64_t c0 = 0; c0 < a.row_bytes; c0 += RS_CH) {
+            const bool in_row = c0 + 16u * piece < a.row_bytes;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if ((uint32_t)t * 8 + sub < gm && in_row)
+                    *reinterpret_cast<rs_vec *>(stage + (t * 8 + sub) * RS_PITCH + 16u * piece) = buf[t];
+            if (sub == 0 && in_row)
+                *reinterpret_cast<rs_vec *>(stage + 64 * RS_PITCH + 16u * piece) = qbuf;
+            if (c0 + RS_CH < a.row_bytes)
+                rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, c0 + RS_CH, gm, piece, sub);
+            wave_sync();
+            if (lane < gm)
+                acc = raw_acc<RT>::add(acc, stage + lane * RS_PITCH, stage + 64 * RS_PITCH,
+                                       (uint32_t)min<uint64_t>(RS_CH, a.row_bytes - c0));
+            wave_sync();
+        }
+        if (lane < gm)
+            kw[g0 + lane] = make_key(raw_acc<RT>::fin(acc), my_id);
+    }
+    for (uint32_t i = m + lane; i < n2; i += WAVE)
+        kw[i] = ~0ull;
+    sort_and_write(a, q, kw, m, n2, lane);
 }
 
 int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
@@ -103,7 +198,26 @@ int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t di
     a.ids = d_out_ids;
     a.dists = d_out_dists;
     const dim3 grid((uint32_t)((nq + 3) / 4)), block(256);
-    if (raw_dtype == NBPG_F32)
+    const bool staged = a.row_bytes % 16 == 0 && q_stride % 16 == 0 &&
+                        ((reinterpret_cast<uintptr_t>(d_raw_base) |
+                          reinterpret_cast<uintptr_t>(d_raw_queries)) & 15) == 0;
+    if (staged) {
+        uint32_t n2 = 1;
+        while (n2 < nc)
+            n2 <<= 1;
+        // 4 waves per group while their LDS fits 64 KiB, else 2
+        const uint32_t per_wave = RS_STAGE + 8 * n2, wpg = 4 * per_wave <= 65536 ? 4 : 2;
+        const dim3 sg((uint32_t)((nq + wpg - 1) / wpg)), sb(WAVE * wpg);
+        const uint32_t lds = wpg * per_wave;
+        if (raw_dtype == NBPG_F32)
+            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_F32>, sg, sb, lds, s, a, n2);
+        else if (raw_dtype == NBPG_U8)
+            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_U8>, sg, sb, lds, s, a, n2);
+        else if (raw_dtype == NBPG_I8)
+            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_I8>, sg, sb, lds, s, a, n2);
+        else
+            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_U4X2>, sg, sb, lds, s, a, n2);
+    } else if (raw_dtype == NBPG_F32)
         hipLaunchKernelGGL(rerank_kernel<NBPG_F32>, grid, block, 0, s, a);
     else if (raw_dtype == NBPG_U8)
         hipLaunchKernelGGL(rerank_kernel<NBPG_U8>, grid, block, 0, s, a);

@@ -76,6 +76,12 @@ def test_rerank_matches_oracle(gpu, raw_kind):
         ei, ed = expected(raw, rq, cand, k, factor, rdt)
         assert (ids.cpu().numpy().view(np.uint32) == ei).all(), (k, factor)
         assert (d.cpu().numpy() == ed).all(), (k, factor)
+    # rows whose size is not a multiple of 16 bytes take the lane-per-row kernel
+    odd, oq = np.ascontiguousarray(raw[:, :61]), np.ascontiguousarray(rq[:, :61])
+    ids, d = N.rerank(torch.from_numpy(odd).cuda(), torch.from_numpy(oq).cuda(), cd[:, :300], 30, 10)
+    ei, ed = expected(odd, oq, cand[:, :300], 30, 10, rdt)
+    assert (ids.cpu().numpy().view(np.uint32) == ei).all()
+    assert (d.cpu().numpy() == ed).all()
     ix.close()
 
 

@@ -196,8 +196,8 @@ def gt_float(x, xq, k, chunk=1 << 17):
 def run_pca(args, log):
     """1M x 960 f32 -> PCA 64: FP64 statistics, host eigendecomposition,
     projection (plain fp32 and bf16 MFMA) with its error against FP64, codes
-    (i8, u4x2, i4x2), a Vamana build (R48, alpha 1.2) and fast search on the
-    codes."""
+    (i8, u4x2, i4x2), a Vamana build (R48, alpha 1.2), fast search on the
+    codes and the same search re-scored with the fp32 rows (rerank mode)."""
     torch = _torch()
     import nbpg_amd as N
     from nbpg_amd import datagen as D
@@ -277,6 +277,16 @@ def run_pca(args, log):
                          "recall@10_vs_float_gt": round(float(
                              np.mean([len(set(ids[i]) & set(gt_f[i, :10])) / 10.0
                                       for i in range(nq)])), 4)})
+            # rerank mode: the beam re-scored with the fp32 rows (all L of
+            # it at the reference's default rerank_factor 100)
+            rr = lambda: N.search_rerank(ix, q_d, x, xq, L, 10, 100, mode=N.SEARCH_FAST,
+                                         fast=(2, 1, 11, 0, 0))
+            rids = rr()[0].cpu().numpy().view(np.uint32)
+            ms_r = events_ms(rr, 5)
+            rows[-1].update({"qps_rerank_f32": round(nq / ms_r * 1e3, 1),
+                             "recall@10_rerank_vs_float_gt": round(float(
+                                 np.mean([len(set(rids[i]) & set(gt_f[i, :10])) / 10.0
+                                          for i in range(nq)])), 4)})
         searches.append({"codes": name, "build_s": round(bs, 2), "per_L": rows})
         log("pca search on %s codes: build %.1fs, %s" % (name, bs, rows))
         ix.close()
