@@ -126,6 +126,15 @@ int nbpg_index_set_graph(nbpg_index *idx, const uint32_t *host_rows, uint64_t fi
                          uint64_t count);
 int nbpg_index_get_graph(const nbpg_index *idx, uint32_t *host_rows, uint64_t first,
                          uint64_t count);
+/* Graph::save / Graph(file) (graph.h:169-221, 339-379) straight from / into
+ * the resident adjacency: [u32 n][u32 R][u32 deg x n][ids].  load needs the
+ * file's n to equal the index's and its R <= the index's R. */
+int nbpg_index_save_graph(const nbpg_index *idx, const char *path);
+int nbpg_index_load_graph(nbpg_index *idx, const char *path);
+/* DFS::repair_isolate_node(G, sp) (dfs.h:101-253) on the resident graph:
+ * nodes unreachable from sp get one reverse edge by the reference's four
+ * rules, in ascending id order; *out_isolated = how many were unreachable. */
+int nbpg_index_repair(nbpg_index *idx, uint32_t sp, uint64_t *out_isolated);
 /* device views for callers that stage their own queries */
 int nbpg_index_info(const nbpg_index *idx, int *dtype, uint64_t *n, uint32_t *dims, uint32_t *R,
                     uint32_t *row_stride);

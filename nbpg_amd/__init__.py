@@ -66,6 +66,9 @@ class ServerStats(C.Structure):
                 ("batch_capacity", U32)]
 
 _SIGS = {
+    "nbpg_index_save_graph": (I32, [VP, C.c_char_p]),
+    "nbpg_index_load_graph": (I32, [VP, C.c_char_p]),
+    "nbpg_index_repair": (I32, [VP, U32, C.POINTER(U64)]),
     "nbpg_last_error": (C.c_char_p, []),
     "nbpg_init": (I32, [I32]),
     "nbpg_device_info": (I32, [C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
@@ -183,6 +186,21 @@ class Index:
         _check(lib().nbpg_index_get_graph(self.h, a.ctypes.data, first, count),
                "nbpg_index_get_graph")
         return a
+
+    def save_graph(self, path):
+        """Write the resident graph as a reference .graph file."""
+        _check(lib().nbpg_index_save_graph(self.h, os.fsencode(path)), "nbpg_index_save_graph")
+
+    def load_graph(self, path):
+        """Read a reference .graph file into the resident graph."""
+        _check(lib().nbpg_index_load_graph(self.h, os.fsencode(path)), "nbpg_index_load_graph")
+
+    def repair(self, sp=0):
+        """Isolated-node repair from entry sp; returns how many nodes were
+        unreachable."""
+        c = U64()
+        _check(lib().nbpg_index_repair(self.h, sp, C.byref(c)), "nbpg_index_repair")
+        return c.value
 
     @property
     def row_stride(self):

@@ -15,9 +15,10 @@
 //                   neighbours without repeats, exact distances, sort and keep
 //                   the R closest (nsw.h:463-511)
 // Passes, shuffles and start-point policies follow build_vamana; the
-// popularity propagation of the CalPopular pass and the isolated-node repair
-// (dfs.h:192-253) are O(N R) sequential host passes over the downloaded
-// graph.  Random start points (fillMergeSP) come from a counter-based
+// popularity propagation of the CalPopular pass is an O(N R) sequential host
+// pass over the downloaded graph (each node's sum feeds later nodes in id
+// order); the isolated-node repair (dfs.h:101-253) runs on the device
+// (graph_io.hip).  Random start points (fillMergeSP) come from a counter-based
 // splitmix64 stream in place of randomInt's random_device-seeded engine.
 #include <float.h>
 #include <math.h>
@@ -597,58 +598,6 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     return NBPG_OK;
 }
 
-// DFS::repair_isolate_node(G, 0) (dfs.h:101-253) on the host copy
-static uint64_t repair_isolated(std::vector<uint32_t> &nbr, std::vector<uint32_t> &deg, uint64_t N,
-                                uint32_t R, uint32_t sp)
-{
-    std::vector<uint8_t> flag(N, 0);
-    std::vector<uint32_t> q;
-    q.reserve(N);
-    flag[sp] = 1;
-    q.push_back(sp);
-    for (size_t h = 0; h < q.size(); ++h) {
-        const uint32_t v = q[h];
-        for (uint32_t j = 0; j < deg[v]; ++j) {
-            const uint32_t u = nbr[(uint64_t)v * R + j];
-            if (!flag[u]) {
-                flag[u] = 1;
-                q.push_back(u);
-            }
-        }
-    }
-    auto add_edge = [&](uint64_t from, uint32_t to) -> bool {
-        if (deg[from] >= R)
-            return false;
-        for (uint32_t k = 0; k < deg[from]; ++k)
-            if (nbr[from * R + k] == to)
-                return true;
-        nbr[from * R + deg[from]] = to;
-        deg[from]++;
-        return true;
-    };
-    uint64_t cnt = 0;
-    std::vector<uint32_t> own;
-    for (uint64_t i = 0; i < N; ++i) {
-        if (flag[i])
-            continue;
-        ++cnt;
-        own.assign(nbr.begin() + i * R, nbr.begin() + i * R + deg[i]);
-        bool done = false;
-        for (uint32_t u : own)
-            if (!done && flag[u] == 1 && add_edge(u, (uint32_t)i))
-                done = true;
-        for (uint32_t u : own)
-            if (!done && add_edge(u, (uint32_t)i))
-                done = true;
-        if (!done && flag[sp] == 1 && add_edge(sp, (uint32_t)i))
-            done = true;
-        for (uint64_t r = 0; r < N && !done; ++r)
-            if (flag[r] == 1 && add_edge(r, (uint32_t)i))
-                done = true;
-    }
-    return cnt;
-}
-
 } // namespace nbpg
 
 using namespace nbpg;
@@ -745,16 +694,8 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
             fprintf(stderr, "[nbpg build] pass %u done\n", i);
     }
     uint64_t iso = 0;
-    if (rc == NBPG_OK && bp->repair_isolate) {
-        std::vector<uint32_t> nbr, deg;
-        rc = download_graph(ix, nbr, deg);
-        if (rc == NBPG_OK) {
-            iso = repair_isolated(nbr, deg, c.N, c.R, 0);
-            if (hipMemcpy(ix->nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(ix->deg, deg.data(), deg.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-                rc = set_error(NBPG_EHIP, "graph upload after repair");
-        }
-    }
+    if (rc == NBPG_OK && bp->repair_isolate) // DFS::repair_isolate_node(G, 0), nsw.h:658-660
+        rc = device_repair(ix, 0, &iso);
     if (out_isolated)
         *out_isolated = iso;
     hipDeviceSynchronize();

@@ -274,7 +274,7 @@ int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, ui
     bool ok = hipMalloc(&ix->vecs, n * ix->stride) == hipSuccess &&
               hipMalloc(&ix->norms, n * 4) == hipSuccess &&
               hipMalloc(&ix->nbr, n * (uint64_t)R * 4) == hipSuccess &&
-              hipMalloc(&ix->deg, n * 4) == hipSuccess && hipMalloc(&ix->counter, 256) == hipSuccess;
+              hipMalloc(&ix->deg, n * 4) == hipSuccess && hipMalloc(&ix->counter, 512) == hipSuccess;
     if (!ok) {
         nbpg_index_destroy(ix);
         return set_error(NBPG_ENOMEM, "index of %llu rows", (unsigned long long)n);
@@ -283,7 +283,7 @@ int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, ui
     HIP_TRY(hipMemset(ix->norms, 0, n * 4));
     HIP_TRY(hipMemset(ix->nbr, 0xFF, n * (uint64_t)R * 4));
     HIP_TRY(hipMemset(ix->deg, 0, n * 4));
-    HIP_TRY(hipMemset(ix->counter, 0, 256));
+    HIP_TRY(hipMemset(ix->counter, 0, 512));
     *out = ix;
     return NBPG_OK;
 }

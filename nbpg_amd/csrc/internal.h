@@ -30,7 +30,10 @@ struct nbpg_index {
     uint32_t *deg;
     void *ws; // grow-only workspace for search/prune scratch
     size_t ws_bytes;
-    uint32_t *counter; // small device buffer: work queue counters and flags
+    // 128 u32 of device flags and counters, each user on its own 128-byte
+    // line: [0] FAST work queue, [32] graph upload check, [40] prune flag,
+    // [48] FAST exit count, [64] BFS / repair counts
+    uint32_t *counter;
 };
 
 namespace nbpg {
@@ -62,6 +65,8 @@ extern DevInfo g_dev;
 
 int ensure_init();
 int ensure_ws(nbpg_index *idx, size_t bytes);
+// DFS::repair_isolate_node on the resident graph (graph_io.hip)
+int device_repair(nbpg_index *ix, uint32_t sp, uint64_t *out_isolated);
 // rerank of candidate rows (rerank.hip); raw query rows at q_stride bytes
 #define RERANK_MAX 1024
 int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,

@@ -134,6 +134,8 @@ def ref():
                                          C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p]
+        R.nbpgr_repair.restype = C.c_uint64
+        R.nbpgr_repair.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32]
         R.nbpgr_prune.restype = C.c_uint64
         R.nbpgr_prune.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                   C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_double]
@@ -273,6 +275,14 @@ def search_exact(rows, base, queries, dtype, raw_base, raw_queries, raw_dtype, L
     if rc != 0:
         raise ValueError("oracle search failed: %d" % rc)
     return ids, d, st
+
+
+def repair(rows, sp=0):
+    """DFS::repair_isolate_node restated: (repaired rows copy, isolated count)."""
+    rows = np.array(rows, dtype=np.uint32, copy=True, order="C")
+    G = Graph(rows.ctypes.data, rows.shape[0], rows.shape[1] - 1)
+    cnt = lib().nbpgo_repair_isolated(C.byref(G), sp)
+    return rows, int(cnt)
 
 
 def build_graph(base, dtype, R, L, alpha=1.2, passes=1, num_split=0, VL=140, min_alpha=1.0,

@@ -23,6 +23,7 @@
 #include "utils/Graph/query_params.h"
 #include "utils/Graph/beamSearch.h"
 #include "utils/Graph/prune.h"
+#include "utils/Graph/dfs.h"
 #include "utils/utilities.h"
 #include "data_proc/base/matrix.h"
 
@@ -216,6 +217,16 @@ int nbpgr_search_exact(int dtype, const uint8_t *base, uint64_t n, uint32_t dims
     case 2: return search_exact_x<uint4_2>(raw_dtype, base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats);
     }
     return -1;
+}
+
+// DFS::repair_isolate_node(G, sp) on reference-layout rows, in place
+uint64_t nbpgr_repair(uint32_t *rows, uint64_t n, uint32_t R, uint32_t sp)
+{
+    auto G = make_graph(rows, n, R);
+    const size_t cnt = DFS::repair_isolate_node(G, (size_t)sp);
+    for (uint64_t i = 0; i < n; ++i)
+        std::memcpy(rows + i * (R + 1), G[i].data(), sizeof(uint32_t) * (R + 1));
+    return cnt;
 }
 
 uint64_t nbpgr_prune(int dtype, const uint8_t *base, uint64_t n, uint32_t dims, uint32_t p,

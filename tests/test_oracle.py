@@ -253,3 +253,52 @@ def test_search_exact_golden(golden_exact, name, cdt, xdt):
                     assert d[q, j] == O.distance(xdt, xb[ids[q, j]], xq[q])
             live = ids[q] != 0xFFFFFFFF
             assert (np.diff(d[q][live]) >= 0).all()
+
+
+def test_repair_golden(golden_repair):
+    """DFS::repair_isolate_node restated: same rows (including stale slots
+    past each degree) and the same isolated count as the reference, on
+    graphs that exercise all four repair steps."""
+    z = golden_repair
+    for g in range(3):
+        sp, cnt = (int(v) for v in z["meta%d" % g])
+        out, c = O.repair(z["in%d" % g], sp)
+        assert c == cnt
+        assert (out == z["out%d" % g]).all()
+
+
+def test_repair_invariants():
+    """Closed-form properties of the repair: rows only grow by appended
+    entries, every node unreachable from the entry has an in-edge afterwards
+    (some reachable row has room here), and new edges point only at
+    unreachable nodes."""
+    rng = np.random.default_rng(9)
+    n, R = 400, 5
+    rows = np.zeros((n, R + 1), np.uint32)
+    for i in range(n):
+        d = int(rng.integers(0, 3))
+        rows[i, 0] = d
+        rows[i, 1:1 + d] = rng.integers(0, n, d)
+
+    def reach(g):
+        seen, stack = {0}, [0]
+        while stack:
+            v = stack.pop()
+            for u in g[v, 1:1 + g[v, 0]]:
+                if int(u) not in seen:
+                    seen.add(int(u))
+                    stack.append(int(u))
+        return seen
+
+    before = reach(rows)
+    out, cnt = O.repair(rows, 0)
+    assert cnt == n - len(before) > 0
+    indeg_new = {}
+    for v in range(n):
+        d0, d1 = int(rows[v, 0]), int(out[v, 0])
+        assert d1 >= d0 and (out[v, 1:1 + d0] == rows[v, 1:1 + d0]).all()
+        for u in out[v, 1 + d0:1 + d1]:
+            indeg_new[int(u)] = indeg_new.get(int(u), 0) + 1
+    assert set(indeg_new) <= set(range(n)) - before
+    has_in = {int(u) for v in range(n) for u in out[v, 1:1 + out[v, 0]] if int(u) != v}
+    assert set(range(n)) - before <= has_in
