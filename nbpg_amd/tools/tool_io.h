@@ -100,23 +100,8 @@ inline Rows read_rows(const std::string &path, uint64_t max_rows = 0)
     return r;
 }
 
-inline void write_graph(const std::string &path, const std::vector<uint32_t> &rows, uint64_t n, uint32_t R)
-{
-    std::ofstream f(path, std::ios::binary);
-    if (!f)
-        throw std::runtime_error("cannot write " + path);
-    const uint32_t hn = (uint32_t)n, hr = R;
-    f.write(reinterpret_cast<const char *>(&hn), 4);
-    f.write(reinterpret_cast<const char *>(&hr), 4);
-    std::vector<uint32_t> deg(n);
-    for (uint64_t i = 0; i < n; ++i)
-        deg[i] = rows[i * (R + 1)];
-    f.write(reinterpret_cast<const char *>(deg.data()), (std::streamsize)(n * 4));
-    for (uint64_t i = 0; i < n; ++i)
-        f.write(reinterpret_cast<const char *>(&rows[i * (R + 1) + 1]), (std::streamsize)deg[i] * 4);
-}
-
-inline std::vector<uint32_t> read_graph(const std::string &path, uint64_t &n, uint32_t &R)
+// the header and degree array of a .graph file (graph.h:182-190)
+inline std::vector<uint32_t> read_graph_degrees(const std::string &path, uint64_t &n, uint32_t &R)
 {
     std::ifstream f(path, std::ios::binary);
     if (!f)
@@ -126,18 +111,13 @@ inline std::vector<uint32_t> read_graph(const std::string &path, uint64_t &n, ui
     f.read(reinterpret_cast<char *>(&hr), 4);
     n = hn;
     R = hr;
-    std::vector<uint32_t> deg(n), rows((size_t)n * (R + 1), 0);
+    std::vector<uint32_t> deg(n);
     f.read(reinterpret_cast<char *>(deg.data()), (std::streamsize)(n * 4));
-    for (uint64_t i = 0; i < n; ++i) {
-        if (deg[i] > R)
-            throw std::runtime_error("degree above R in " + path);
-        rows[i * (R + 1)] = deg[i];
-        f.read(reinterpret_cast<char *>(&rows[i * (R + 1) + 1]), (std::streamsize)deg[i] * 4);
-    }
     if (!f)
         throw std::runtime_error("short file " + path);
-    return rows;
+    return deg;
 }
+
 
 inline void write_gt(const std::string &path, uint32_t nq, uint32_t k, const std::vector<uint32_t> &ids,
                      const std::vector<float> &d)

@@ -169,42 +169,45 @@ int main(int argc, char **argv)
         nbpg_index *ix = nullptr;
         uint64_t n = base.n;
         uint32_t gR = R;
-        std::vector<uint32_t> rows;
         double build_s = 0;
         uint64_t isolated = 0;
+        // the graph moves between the file and the device without a host
+        // copy of its rows (nbpg_index_load_graph / nbpg_index_save_graph)
         if (file_exists(gfile) && !f.has("rebuild")) {
-            rows = read_graph(gfile, n, gR);
+            read_graph_degrees(gfile, n, gR);
             if (n != base.n)
                 throw std::runtime_error("graph and base sizes differ");
             nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, gR), "nbpg_index_create");
             nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
-            nbpg_ok(nbpg_index_set_graph(ix, rows.data(), 0, n), "set_graph");
+            nbpg_ok(nbpg_index_load_graph(ix, gfile.c_str()), "load_graph");
         } else {
             nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, R), "nbpg_index_create");
             nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
             if (!f.str("pre_index_path").empty()) { // Vamana.cpp:174-177
+                const std::string pf = f.str("pre_index_path") + ".graph";
                 uint64_t pn = 0;
                 uint32_t pR = 0;
-                auto pre = read_graph(f.str("pre_index_path") + ".graph", pn, pR);
+                read_graph_degrees(pf, pn, pR);
                 if (pn != n || pR != R)
                     throw std::runtime_error("pre-built graph shape differs");
-                nbpg_ok(nbpg_index_set_graph(ix, pre.data(), 0, n), "set_graph");
+                nbpg_ok(nbpg_index_load_graph(ix, pf.c_str()), "load_graph");
             }
             const auto t0 = std::chrono::steady_clock::now();
             nbpg_ok(nbpg_build(ix, &p, &isolated), "nbpg_build");
             build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            rows.assign((size_t)n * (R + 1), 0);
-            nbpg_ok(nbpg_index_get_graph(ix, rows.data(), 0, n), "get_graph");
             std::cout << "Writing graph with " << n << " points and max degree " << R << std::endl;
-            write_graph(gfile, rows, n, R);
+            nbpg_ok(nbpg_index_save_graph(ix, gfile.c_str()), "save_graph");
         }
         // graph statistics (parse_results.h:7-23, Vamana.cpp:91-111)
         uint64_t sum = 0;
         uint32_t maxd = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            const uint32_t d = rows[i * (gR + 1)];
-            sum += d;
-            maxd = std::max(maxd, d);
+        {
+            uint64_t fn = 0;
+            uint32_t fR = 0;
+            for (uint32_t d : read_graph_degrees(gfile, fn, fR)) {
+                sum += d;
+                maxd = std::max(maxd, d);
+            }
         }
         const double avg_deg = (double)sum / (double)n;
         std::ostringstream params;
