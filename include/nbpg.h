@@ -37,6 +37,11 @@ extern "C" {
 #define NBPG_I4X2 3 /* .i4bin: distance identical to U4X2 (point.h:107-120) */
 #define NBPG_F32 4  /* .fbin / .bin: PCA input and raw rerank rows (matrix.h:27-57) */
 
+/* ---- distance: DistanceMetric (point.h:20-24), chosen by --use_mips
+ *      (Vamana.cpp:76, CompGT.cpp:147) ---- */
+#define NBPG_METRIC_L2 0   /* squared Euclidean (the default) */
+#define NBPG_METRIC_MIPS 1 /* negated inner product, mips_distance (point.h:46-156) */
+
 /* ---- search modes ---- */
 #define NBPG_SEARCH_REF 0  /* one query per workgroup, reference-exact IDs */
 #define NBPG_SEARCH_FAST 1 /* persistent, one wavefront per query, LDS filter + beam */
@@ -54,6 +59,9 @@ extern "C" {
  * stale-maximum step (CompGT.cpp:82-84) can evict a row of the true top-k;
  * this mode reproduces both. */
 #define NBPG_GT_REFTIES 0x100
+/* OR into the mode: inner-product ground truth (CompGT --use_mips,
+ * CompGT.cpp:51, 147), any engine */
+#define NBPG_GT_MIPS 0x200
 
 /* ---- PCA: projection forms and code types.  The code type is what the
  *      reference derives from the output extension (PCACompress.cpp:25-37):
@@ -116,6 +124,9 @@ int nbpg_device_info(int *cu_count, int *lds_per_group, int *l2_bytes);
  * and Graph(max_degree, n) / Graph(file) (graph.h:163-221). */
 int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, uint32_t R);
 int nbpg_index_destroy(nbpg_index *idx);
+/* the index's distance (NBPG_METRIC_L2 at creation): every search, prune,
+ * build and distance call on it uses this metric afterwards */
+int nbpg_index_set_metric(nbpg_index *idx, int metric);
 /* upload rows [first, first+count) from host memory with the given stride
  * (the reference's 64-B padded rows, point_range.h:51-56, or packed) */
 int nbpg_index_set_points(nbpg_index *idx, const void *host_rows, uint64_t host_stride,
@@ -199,7 +210,8 @@ int nbpg_robust_prune_batch(const nbpg_index *idx, const uint32_t *d_nodes, uint
 
 /* ---- brute-force ground truth: compute_groundtruth (CompGT.cpp:29-134)
  * per query the k rows sorted by (dist, id), rows packed (dims bytes each).
- * mode = NBPG_GT_PLAIN or NBPG_GT_MFMA, optionally | NBPG_GT_REFTIES. */
+ * mode = NBPG_GT_PLAIN or NBPG_GT_MFMA, optionally | NBPG_GT_REFTIES and
+ * | NBPG_GT_MIPS. */
 int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries, uint64_t nq,
                 uint32_t dims, uint32_t k, int mode, uint32_t *d_out_ids, float *d_out_dists,
                 void *stream);
@@ -209,10 +221,11 @@ int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void *d_queries
  * (nq rows of cand_stride ids, first n_cand used, an id >= n ends a row).
  * The first min(k * rerank_factor, n_cand) candidates (at most 1024) are
  * re-scored against the full-precision rows (raw_dtype NBPG_F32 / NBPG_U8 /
- * NBPG_I8, dims elements each, packed) and the k nearest by (dist, id) are
- * written, nq x k; missing slots get NBPG_SENT / +inf.  f32 distances are the
- * reference's sequential fp32 sum (point.h:137-146). */
-int nbpg_rerank(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+ * NBPG_I8 / NBPG_U4X2 / NBPG_I4X2, dims elements each, packed) under
+ * `metric` (NBPG_METRIC_L2 / NBPG_METRIC_MIPS) and the k nearest by
+ * (dist, id) are written, nq x k; missing slots get NBPG_SENT / +inf.  f32
+ * distances are the reference's sequential fp32 sums (point.h:137-156). */
+int nbpg_rerank(int raw_dtype, int metric, const void *d_raw_base, uint64_t n, uint32_t dims,
                 const void *d_raw_queries, uint64_t nq, const uint32_t *d_cand,
                 uint32_t cand_stride, uint32_t n_cand, uint32_t k, uint32_t rerank_factor,
                 uint32_t *d_out_ids, float *d_out_dists, void *stream);

@@ -21,7 +21,8 @@ CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
 CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
                ".i4bin": CODE_I4}
 SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER = 0, 1, 0x10
-GT_PLAIN, GT_MFMA, GT_REFTIES = 0, 1, 0x100
+METRIC_L2, METRIC_MIPS = 0, 1
+GT_PLAIN, GT_MFMA, GT_REFTIES, GT_MIPS = 0, 1, 0x100, 0x200
 SENT = 0xFFFFFFFF
 DTYPE_OF_EXT = {".u8bin": U8, ".i8bin": I8, ".u4bin": U4X2, ".i4bin": I4X2}
 
@@ -95,7 +96,8 @@ _SIGS = {
     "nbpg_server_wait": (I32, [VP, U64, VP, VP, C.POINTER(DBL)]),
     "nbpg_server_stats": (I32, [VP, I32, C.POINTER(ServerStats)]),
     "nbpg_server_destroy": (I32, [VP]),
-    "nbpg_rerank": (I32, [I32, VP, U64, U32, VP, U64, VP, U32, U32, U32, U32, VP, VP, VP]),
+    "nbpg_rerank": (I32, [I32, I32, VP, U64, U32, VP, U64, VP, U32, U32, U32, U32, VP, VP, VP]),
+    "nbpg_index_set_metric": (I32, [VP, I32]),
     "nbpg_pca_project": (I32, [I32, VP, U64, U32, VP, VP, U32, I32, I32, VP, VP, VP]),
     "nbpg_pca_quantize": (I32, [VP, U64, U32, I32, VP, VP]),
     "nbpg_pca_stats_f64": (I32, [I32, VP, U64, U32, VP, VP, VP]),
@@ -186,6 +188,10 @@ class Index:
         _check(lib().nbpg_index_get_graph(self.h, a.ctypes.data, first, count),
                "nbpg_index_get_graph")
         return a
+
+    def set_metric(self, metric):
+        """METRIC_L2 (the default) or METRIC_MIPS for every call on this index."""
+        _check(lib().nbpg_index_set_metric(self.h, metric), "nbpg_index_set_metric")
 
     def save_graph(self, path):
         """Write the resident graph as a reference .graph file."""
@@ -370,7 +376,8 @@ def compgt(base_dev, q_dev, k, dtype=U8, mode=GT_PLAIN):
     return ids, d
 
 
-def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100, raw_dtype=None):
+def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100, raw_dtype=None,
+           metric=None):
     """Re-score each query's frontier (cand_dev: nq x C ids, e.g. the ids of a
     search over codes with k = L) with the full-precision rows; returns
     (ids, dists) nq x k."""
@@ -382,7 +389,8 @@ def rerank(raw_base_dev, raw_q_dev, cand_dev, k, rerank_factor=100, raw_dtype=No
     ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
     d = torch.empty((nq, k), dtype=torch.float32, device="cuda")
     rdt = _x_dtype(raw_base_dev) if raw_dtype is None else raw_dtype
-    _check(lib().nbpg_rerank(rdt, _ptr(raw_base_dev), n, dims, _ptr(raw_q_dev),
+    _check(lib().nbpg_rerank(rdt, METRIC_L2 if metric is None else metric, _ptr(raw_base_dev), n,
+                             dims, _ptr(raw_q_dev),
                              nq, _ptr(cand_dev), cand_dev.stride(0), C, k, rerank_factor,
                              _ptr(ids), _ptr(d),
                              _stream()), "nbpg_rerank")

@@ -55,6 +55,7 @@ struct InsertArgs {
     const uint32_t *deg;
     uint64_t n, m;
     uint32_t stride, R;
+    uint32_t mips; // inner-product metric
     const uint32_t *inserts;
     uint64_t floor, bs;
     int sp_mode;
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
     const RefLayout lo = ref_layout(a.L, a.R, a.VL, a.H);
     uint64_t *ex = reinterpret_cast<uint64_t *>(st + lo.total);
     uint64_t *cp = ex + a.R;
-    const GraphView g = {a.vecs, a.norms, a.nbr, a.stride, a.R};
+    const GraphView g = {a.vecs, a.norms, a.nbr, a.stride, a.R, a.mips};
     const uint32_t nsp = a.sp_mode == SP_MERGE ? a.num_split : 1;
     for (uint64_t j = blockIdx.x; j < a.bs; j += gridDim.x) {
         const uint32_t index = a.inserts[a.floor + j];
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
             if (fresh) {
                 const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)av * a.stride);
                 ex[fl + prefix_count(um)] =
-                    make_key(dist_from<DT>(a.norms[av], nqn, dot_row<DT, NCH>(r, qs)), av);
+                    make_key(dist_from<DT>(a.norms[av], nqn, dot_row<DT, NCH>(r, qs), a.mips), av);
             }
             fl += (uint32_t)__popcll(um);
             wave_sync();
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
         uint32_t comps = 0;
         const uint32_t sel =
             wave_prune<DT, NCH>(cp, mm, index, a.R, (double)af, (double)a.cos_f, a.vecs, a.stride,
-                                a.norms, a.p_ids + j * a.R, a.p_d + j * a.R, comps, lane);
+                                a.norms, a.p_ids + j * a.R, a.p_d + j * a.R, comps, lane, a.mips);
         if (lane == 0)
             a.p_cnt[j] = sel;
         wave_sync();
@@ -227,18 +228,6 @@ __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
-// order-preserving map of a float (possibly negative, from the interpolation
-// of eval_dist_of_new_neighbors) to u32; -0 maps like +0
-__device__ __forceinline__ uint32_t ord_f(float f)
-{
-    uint32_t b = __float_as_uint(f == 0.f ? 0.f : f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float unord_f(uint32_t o)
-{
-    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
-}
-
 #define REV_CAP 512
 
 struct ReverseArgs {
@@ -250,6 +239,7 @@ struct ReverseArgs {
     const uint32_t *uniq, *counts, *offsets, *num_runs;
     const uint64_t *key2;
     uint64_t max_groups;
+    uint32_t mips;
 };
 
 template <int DT, int NCH>
@@ -307,7 +297,7 @@ __global__ __launch_bounds__(256) void reverse_kernel(ReverseArgs a)
     to_uniform<NCH>(tv, ts);
     auto exact = [&](uint32_t id) -> float {
         const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)id * a.stride);
-        return dist_from<DT>(a.norms[id], nt, dot_row<DT, NCH>(r, ts));
+        return dist_from<DT>(a.norms[id], nt, dot_row<DT, NCH>(r, ts), a.mips);
     };
     const uint32_t end = min(pre + a.top_degree, cs);
     for (uint32_t t = pre + lane; t < end; t += WAVE) {
@@ -462,6 +452,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     memset(&a, 0, sizeof(a));
     a.vecs = ix->vecs;
     a.norms = ix->norms;
+    a.mips = ix->mips;
     a.nbr = ix->nbr;
     a.deg = ix->deg;
     a.n = c.N;
@@ -491,6 +482,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     ReverseArgs ra;
     ra.vecs = ix->vecs;
     ra.norms = ix->norms;
+    ra.mips = ix->mips;
     ra.nbr = ix->nbr;
     ra.deg = ix->deg;
     ra.stride = ix->stride;

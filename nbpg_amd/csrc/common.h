@@ -94,10 +94,14 @@ __device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row,
     return acc;
 }
 
-// exact reference distance as float: (float)(int32) sum of squares
+// exact reference distance as float: (float)(int32) sum of squares, or
+// under the inner-product metric (mips != 0) the negated integer dot product
+// (mips_distance, point.h:46-135)
 template <int DT>
-__device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t dot)
+__device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t dot, uint32_t mips = 0)
 {
+    if (mips)
+        return -(float)(int32_t)(dot * dot_traits<DT>::scale);
     uint32_t s = nx + nq - 2u * dot; // wraps back into range: true value < 2^31
     return (float)(int32_t)(s * dot_traits<DT>::scale);
 }
@@ -106,7 +110,7 @@ __device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t do
 // chunks (one more dot per word), so a distance is exactly one row gather
 template <int DT, int NCH>
 __device__ __forceinline__ float dist_row(const uint4 *__restrict__ row, const uint32_t (&q)[4 * NCH],
-                                         uint32_t nq)
+                                         uint32_t nq, uint32_t mips = 0)
 {
     uint4 x[NCH];
 #pragma unroll
@@ -124,13 +128,14 @@ __device__ __forceinline__ float dist_row(const uint4 *__restrict__ row, const u
         xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
         xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
     }
-    return dist_from<DT>(xx, nq, xq);
+    return dist_from<DT>(xx, nq, xq, mips);
 }
 
 // same, with the query read from LDS one 16-byte chunk at a time (every lane
 // reads the same address: a broadcast), so the query costs no registers
 template <int DT, int NCH>
-__device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, const uint4 *q, uint32_t nq)
+__device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, const uint4 *q, uint32_t nq,
+                                             uint32_t mips = 0)
 {
     uint4 x[NCH];
 #pragma unroll
@@ -149,7 +154,7 @@ __device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, con
         xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
         xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
     }
-    return dist_from<DT>(xx, nq, xq);
+    return dist_from<DT>(xx, nq, xq, mips);
 }
 
 template <int DT, int NCH>
@@ -177,16 +182,28 @@ __device__ __forceinline__ uint64_t hash64_2(uint64_t x)
 }
 
 // ---------------------------------------------------------------- keys
+// order-preserving map of a float to u32 (inner-product distances are
+// negative): positive values get the top bit set, negative ones are
+// inverted; -0 maps like +0, as the two compare equal
+__device__ __forceinline__ uint32_t ord_f(float f)
+{
+    const uint32_t b = __float_as_uint(f == 0.f ? 0.f : f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f(uint32_t o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
 // (dist, id) records ordered as the reference's id_dist_less
-// (query_params.h:115-119).  Distances are non-negative floats, whose bit
-// patterns order like the values, so (bits(dist) << 32 | id) orders as
-// (dist asc, id asc) under plain u64 comparison.
+// (query_params.h:115-119): (ord(dist) << 32 | id) orders as (dist asc,
+// id asc) under plain u64 comparison; ~0 (EMPTY) is above every record.
 __device__ __forceinline__ uint64_t make_key(float d, uint32_t id)
 {
-    return ((uint64_t)__float_as_uint(d) << 32) | id;
+    return ((uint64_t)ord_f(d) << 32) | id;
 }
 __device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)k; }
-__device__ __forceinline__ float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ float key_dist(uint64_t k) { return unord_f((uint32_t)(k >> 32)); }
 
 // ---------------------------------------------------------------- wave
 __device__ __forceinline__ uint32_t lane_id()

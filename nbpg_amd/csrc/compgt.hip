@@ -29,6 +29,7 @@ struct GtArgs {
     uint64_t slice_rows;
     uint64_t row_stride; // bytes between base rows (dims, or a multiple to sample)
     uint64_t *partial;   // [nq][slices][k] keys
+    uint32_t mips;       // inner-product metric
 };
 
 // ascending bitonic sort of a[0:n2) by the whole workgroup
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(GT_THREADS) void compgt_plain_kernel(GtArgs a)
 #pragma unroll
                 for (int w = 0; w < NW; ++w)
                     acc = dot_traits<DT>::dot(x[w], qs[q][w], acc);
-                const uint64_t key = make_key(dist_from<DT>(nx, qn[q], acc), (uint32_t)r);
+                const uint64_t key = make_key(dist_from<DT>(nx, qn[q], acc, a.mips), (uint32_t)r);
                 if (key < thr[q]) {
                     const uint32_t p = atomicAdd(&cnt[q], 1u);
                     buf[q][p] = key;
@@ -209,17 +210,18 @@ __global__ __launch_bounds__(256) void compgt_merge_kernel(const uint64_t *__res
 }
 
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                uint32_t dims, uint32_t k, uint32_t *out_ids, float *out_dists, hipStream_t stream);
+                uint32_t dims, uint32_t k, uint32_t mips, uint32_t *out_ids, float *out_dists,
+                hipStream_t stream);
 
 int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                   uint32_t dims, uint32_t k, bool use_mfma, uint32_t *ids, float *dists,
-                   hipStream_t s);
+                   uint32_t dims, uint32_t k, bool use_mfma, uint32_t mips, uint32_t *ids,
+                   float *dists, hipStream_t s);
 
 // plain scan + slice merge into ids / dists (dists may be NULL).  Base row r
 // is at base + r * row_stride and is reported as id r.
 int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
                         const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
-                        uint32_t *ids, float *dists, hipStream_t s)
+                        uint32_t mips, uint32_t *ids, float *dists, hipStream_t s)
 {
     const uint32_t nch = nch_for_bytes(dims);
     const uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
@@ -242,6 +244,7 @@ int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row
     a.slice_rows = slice_rows;
     a.row_stride = row_stride;
     a.partial = partial;
+    a.mips = mips;
     int rc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
         hipLaunchKernelGGL((compgt_plain_kernel<DT.value, NCH.value>),
                            dim3((uint32_t)tiles, (uint32_t)slices), dim3(GT_THREADS), 0, s, a);
@@ -285,6 +288,8 @@ extern "C" int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void
     if (nq == 0)
         return NBPG_OK;
     hipStream_t s = as_stream(stream);
+    const uint32_t mips = (mode & NBPG_GT_MIPS) ? 1u : 0u;
+    mode &= ~NBPG_GT_MIPS;
     if (mode & NBPG_GT_REFTIES) {
         const int engine = mode & ~NBPG_GT_REFTIES;
         if (engine == NBPG_GT_MFMA &&
@@ -294,14 +299,14 @@ extern "C" int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void
         if (engine != NBPG_GT_MFMA && engine != NBPG_GT_PLAIN)
             return set_error(NBPG_EINVAL, "compgt mode %d", mode);
         return compgt_refrule(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq,
-                              dims, k, engine == NBPG_GT_MFMA, d_out_ids, d_out_dists, s);
+                              dims, k, engine == NBPG_GT_MFMA, mips, d_out_ids, d_out_dists, s);
     }
     if (mode == NBPG_GT_MFMA)
         return compgt_mfma(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq, dims,
-                           k, d_out_ids, d_out_dists, s);
+                           k, mips, d_out_ids, d_out_dists, s);
     if (mode == NBPG_GT_PLAIN)
         return compgt_plain_launch(dtype, (const uint8_t *)d_base, n, dims,
-                                   (const uint8_t *)d_queries, nq, dims, k, d_out_ids,
+                                   (const uint8_t *)d_queries, nq, dims, k, mips, d_out_ids,
                                    d_out_dists, s);
     return set_error(NBPG_EINVAL, "compgt mode %d", mode);
 }

@@ -28,11 +28,13 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
                         const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
-                        uint32_t *ids, float *dists, hipStream_t s);
+                        uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
 
-// |x'|^2 per row, x' = x ^ 0x80 (u8) or x (i8), over dims bytes
+// per row, with x' = x ^ 0x80 = x - 128 (u8) or x (i8), over dims bytes:
+// L2: |x'|^2.  Inner product: half of what turns <x',q'> back into <x,q>,
+// <x,q> = <x',q'> + A(x) + A(q) with A(x) = 128 sum(x') + 8192 dims (u8), 0 (i8)
 __global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, uint32_t dims,
-                                int is_u8, uint32_t *__restrict__ out)
+                                int is_u8, uint32_t mips, uint32_t *__restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -41,8 +43,10 @@ __global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, ui
     int acc = 0;
     for (uint32_t t = 0; t < dims / 4; ++t) {
         const uint32_t v = is_u8 ? (w[t] ^ 0x80808080u) : w[t];
-        acc = __builtin_amdgcn_sdot4((int)v, (int)v, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)v, mips ? 0x01010101 : (int)v, acc, false);
     }
+    if (mips)
+        acc = is_u8 ? 128 * acc + 8192 * (int)dims : 0;
     out[i] = (uint32_t)acc;
 }
 
@@ -90,7 +94,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
     const uint8_t *__restrict__ base, uint64_t n, const uint8_t *__restrict__ queries, uint64_t nq,
     uint32_t dims, int is_u8, const uint32_t *__restrict__ nx, const uint32_t *__restrict__ nqv,
     const uint64_t *__restrict__ thr, uint64_t slice_rows, uint32_t *__restrict__ cnt,
-    uint64_t *__restrict__ cand)
+    uint64_t *__restrict__ cand, uint32_t mips)
 {
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint32_t row16 = lane % 16, kq = lane / 16;
@@ -147,8 +151,9 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
                 c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t d = qn[t][r] + xn - 2u * (uint32_t)c[r];
-                const uint64_t key = make_key((float)(int32_t)d, (uint32_t)row);
+                const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
+                                     : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
+                const uint64_t key = make_key(d, (uint32_t)row);
                 if (rv && key < th[t][r]) {
                     const uint64_t qr = q0 + 16 * t + 4 * kq + r;
                     const uint32_t p = atomicAdd(&cnt[qr], 1u);
@@ -203,7 +208,8 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
 }
 
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                uint32_t dims, uint32_t k, uint32_t *out_ids, float *out_dists, hipStream_t s)
+                uint32_t dims, uint32_t k, uint32_t mips, uint32_t *out_ids, float *out_dists,
+                hipStream_t s)
 {
     if (dtype != NBPG_U8 && dtype != NBPG_I8)
         return set_error(NBPG_EUNSUP, "MFMA ground truth takes u8 or i8 rows");
@@ -243,7 +249,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         cleanup();
         return set_error(NBPG_ENOMEM, "MFMA ground-truth workspace");
     }
-    int rc = compgt_plain_launch(dtype, base, S, step * dims, queries, nq, dims, k, tid, td, s);
+    int rc = compgt_plain_launch(dtype, base, S, step * dims, queries, nq, dims, k, mips, tid, td, s);
     if (rc != NBPG_OK) {
         cleanup();
         return rc;
@@ -251,9 +257,9 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     hipLaunchKernelGGL(mf_thresholds_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
                        td, nq, k, thr, cnt, ovf);
     hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, base, n,
-                       dims, is_u8, nx);
+                       dims, is_u8, mips, nx);
     hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
-                       queries, nq, dims, is_u8, nqv);
+                       queries, nq, dims, is_u8, mips, nqv);
     // 2. MFMA scan: query tiles x base slices
     const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
     const uint64_t tiles = (nq + qpb - 1) / qpb;
@@ -263,10 +269,10 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     slices = (n + slice_rows - 1) / slice_rows;
     dim3 grid((uint32_t)tiles, (uint32_t)slices), block(64 * MF_WAVES);
     switch (KS) {
-    case 1: hipLaunchKernelGGL((mf_scan_kernel<1>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
-    case 2: hipLaunchKernelGGL((mf_scan_kernel<2>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
-    case 3: hipLaunchKernelGGL((mf_scan_kernel<3>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
-    default: hipLaunchKernelGGL((mf_scan_kernel<4>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand); break;
+    case 1: hipLaunchKernelGGL((mf_scan_kernel<1>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
+    case 2: hipLaunchKernelGGL((mf_scan_kernel<2>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
+    case 3: hipLaunchKernelGGL((mf_scan_kernel<3>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
+    default: hipLaunchKernelGGL((mf_scan_kernel<4>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
     }
     if (hipGetLastError() != hipSuccess) {
         cleanup();
@@ -299,7 +305,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
             const uint64_t tb = (uint64_t)m * dims, tk = (uint64_t)m * k;
             hipLaunchKernelGGL(mf_gather_kernel, dim3((uint32_t)((tb + 255) / 256)), dim3(256), 0,
                                s, queries, idx, m, dims, rows);
-            rc = compgt_plain_launch(dtype, base, n, dims, rows, m, dims, k, cids, cd, s);
+            rc = compgt_plain_launch(dtype, base, n, dims, rows, m, dims, k, mips, cids, cd, s);
             if (rc == NBPG_OK)
                 hipLaunchKernelGGL(mf_scatter_kernel, dim3((uint32_t)((tk + 255) / 256)),
                                    dim3(256), 0, s, cids, cd, idx, m, k, out_ids, out_dists);

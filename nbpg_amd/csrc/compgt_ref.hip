@@ -37,6 +37,7 @@ struct RefGtArgs {
     uint32_t dims, k;
     uint32_t *ids;
     float *dists;
+    uint32_t mips; // inner-product metric
 };
 
 __device__ __forceinline__ uint64_t wave_max64(uint64_t v)
@@ -57,7 +58,7 @@ __device__ __forceinline__ uint64_t first_max(const float *sd, uint32_t k, float
     uint64_t best = 0;
     for (uint32_t l = lane; l < k; l += WAVE) {
         const float v = l == 0 ? v0 : sd[l];
-        const uint64_t key = ((uint64_t)__float_as_uint(v) << 32) | (~l & 0xFFFFFFFFu);
+        const uint64_t key = ((uint64_t)ord_f(v) << 32) | (~l & 0xFFFFFFFFu);
         best = key > best ? key : best;
     }
     return wave_max64(best);
@@ -85,12 +86,30 @@ __device__ __forceinline__ void rg_wave_bitonic(uint64_t *a, uint32_t n2, uint32
 template <int KS, int QT> struct MfmaTile {
     static constexpr int RQ = 16 * QT;
     v4i a[QT][KS];
-    uint32_t qn[QT][4];
-    uint32_t xmask;
+    uint32_t qn[QT][4]; // |q'|^2, or A(q) under the inner product (compgt_mfma.hip)
+    uint32_t xmask, is_u8_, mips;
+
+    // |v|^2 of the row's words, or sum(v) for the inner product
+    __device__ int part_of(const v4i &v, int part) const
+    {
+        const int o = 0x01010101;
+        part = __builtin_amdgcn_sdot4(v.x, mips ? o : v.x, part, false);
+        part = __builtin_amdgcn_sdot4(v.y, mips ? o : v.y, part, false);
+        part = __builtin_amdgcn_sdot4(v.z, mips ? o : v.z, part, false);
+        part = __builtin_amdgcn_sdot4(v.w, mips ? o : v.w, part, false);
+        return part;
+    }
+    // <x,q> = <x',q'> + A(x) + A(q), A = 128 sum(x') + 8192 dims for u8, 0 for i8
+    __device__ int a_of(int sum, uint32_t dims) const
+    {
+        return is_u8_ ? 128 * sum + 8192 * (int)dims : 0;
+    }
 
     __device__ void init(const RefGtArgs &g, uint64_t q0, uint32_t is_u8, void *, uint32_t lane)
     {
         xmask = is_u8 ? 0x80808080u : 0u;
+        is_u8_ = is_u8;
+        mips = g.mips;
         const uint32_t row16 = lane % 16, kq = lane / 16;
 #pragma unroll
         for (int t = 0; t < QT; ++t) {
@@ -106,13 +125,12 @@ template <int KS, int QT> struct MfmaTile {
                             (int)(u.w ^ xmask)};
                 }
                 a[t][s] = v;
-                part = __builtin_amdgcn_sdot4(v.x, v.x, part, false);
-                part = __builtin_amdgcn_sdot4(v.y, v.y, part, false);
-                part = __builtin_amdgcn_sdot4(v.z, v.z, part, false);
-                part = __builtin_amdgcn_sdot4(v.w, v.w, part, false);
+                part = part_of(v, part);
             }
             part += __shfl_xor(part, 16);
-            part += __shfl_xor(part, 32); // |q'|^2 of query 16t + row16
+            part += __shfl_xor(part, 32); // |q'|^2 (or sum(q')) of query 16t + row16
+            if (mips)
+                part = a_of(part, g.dims);
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 qn[t][r] = (uint32_t)__shfl(part, 4 * kq + r);
@@ -140,13 +158,12 @@ template <int KS, int QT> struct MfmaTile {
                             (int)(u.w ^ xmask)};
                 }
                 x[s] = v;
-                part = __builtin_amdgcn_sdot4(v.x, v.x, part, false);
-                part = __builtin_amdgcn_sdot4(v.y, v.y, part, false);
-                part = __builtin_amdgcn_sdot4(v.z, v.z, part, false);
-                part = __builtin_amdgcn_sdot4(v.w, v.w, part, false);
+                part = part_of(v, part);
             }
             part += __shfl_xor(part, 16);
-            part += __shfl_xor(part, 32); // |x'|^2 of this lane's row
+            part += __shfl_xor(part, 32); // |x'|^2 (or sum(x')) of this lane's row
+            if (mips)
+                part = a_of(part, g.dims);
 #pragma unroll
             for (int t = 0; t < QT; ++t) {
                 v4i c = {0, 0, 0, 0};
@@ -155,9 +172,10 @@ template <int KS, int QT> struct MfmaTile {
                     c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], x[s], c, 0, 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const uint32_t d = qn[t][r] + (uint32_t)part - 2u * (uint32_t)c[r];
-                    D[(16 * t + 4 * kq + r) * RG_TS + col] =
-                        rv ? (float)(int32_t)d : __builtin_huge_valf();
+                    const float d =
+                        mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + (uint32_t)part)
+                             : (float)(int32_t)(qn[t][r] + (uint32_t)part - 2u * (uint32_t)c[r]);
+                    D[(16 * t + 4 * kq + r) * RG_TS + col] = rv ? d : __builtin_huge_valf();
                 }
             }
         }
@@ -213,7 +231,7 @@ template <int DT, int NCH, int QT> struct DotTile {
 #pragma unroll
             for (int w = 0; w < NW; ++w)
                 acc = dot_traits<DT>::dot(x[w], qs[q * NW + w], acc);
-            D[q * RG_TS + col] = rv ? dist_from<DT>(nx, qn[q], acc) : __builtin_huge_valf();
+            D[q * RG_TS + col] = rv ? dist_from<DT>(nx, qn[q], acc, g.mips) : __builtin_huge_valf();
         }
     }
 };
@@ -265,7 +283,7 @@ __global__ __launch_bounds__(RG_THREADS) void compgt_refrule_kernel(RefGtArgs g,
                     wave_sync();
                     if (r0 + WAVE >= k) {
                         const uint64_t key = first_max(qd, k, qd[0], lane);
-                        topd = __uint_as_float((uint32_t)(key >> 32));
+                        topd = unord_f((uint32_t)(key >> 32));
                         topp = ~(uint32_t)key;
                     }
                 }
@@ -281,7 +299,7 @@ __global__ __launch_bounds__(RG_THREADS) void compgt_refrule_kernel(RefGtArgs g,
                     }
                     wave_sync();
                     const uint64_t key = first_max(qd, k, nd, lane);
-                    topd = __uint_as_float((uint32_t)(key >> 32));
+                    topd = unord_f((uint32_t)(key >> 32));
                     topp = ~(uint32_t)key;
                     m = ballot(row >= k && v < topd && lane > p);
                 }
@@ -332,10 +350,11 @@ static int launch_refrule(const RefGtArgs &g, uint32_t is_u8, size_t extra, hipS
 
 // use_mfma: u8 / i8 rows of a multiple of 16 bytes (<= 256), 16-byte aligned
 int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
-                   uint32_t dims, uint32_t k, bool use_mfma, uint32_t *ids, float *dists,
-                   hipStream_t s)
+                   uint32_t dims, uint32_t k, bool use_mfma, uint32_t mips, uint32_t *ids,
+                   float *dists, hipStream_t s)
 {
     RefGtArgs g;
+    g.mips = mips;
     g.base = base;
     g.queries = queries;
     g.n = n;

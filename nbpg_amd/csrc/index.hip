@@ -142,7 +142,7 @@ template <int DT, int NCH>
 __global__ void l2_batch_kernel(const uint8_t *__restrict__ vecs, uint32_t stride,
                                 const uint32_t *__restrict__ norms, const uint8_t *__restrict__ q,
                                 uint32_t qbytes, const uint32_t *__restrict__ ids, uint64_t n,
-                                uint64_t nrows, float *__restrict__ out)
+                                uint64_t nrows, float *__restrict__ out, uint32_t mips)
 {
     uint4 qv[NCH];
     const uint32_t *qw = reinterpret_cast<const uint32_t *>(q);
@@ -177,7 +177,7 @@ __global__ void l2_batch_kernel(const uint8_t *__restrict__ vecs, uint32_t strid
         return;
     }
     const uint4 *r = reinterpret_cast<const uint4 *>(vecs + (uint64_t)id * stride);
-    out[i] = dist_from<DT>(norms[id], nq, dot_row<DT, NCH>(r, qs));
+    out[i] = dist_from<DT>(norms[id], nq, dot_row<DT, NCH>(r, qs), mips);
 }
 
 // arbitrary packed rows: one thread per pair, plain byte loop
@@ -285,6 +285,16 @@ int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, ui
     HIP_TRY(hipMemset(ix->deg, 0, n * 4));
     HIP_TRY(hipMemset(ix->counter, 0, 512));
     *out = ix;
+    return NBPG_OK;
+}
+
+int nbpg_index_set_metric(nbpg_index *ix, int metric)
+{
+    if (!ix)
+        return set_error(NBPG_EINVAL, "null argument");
+    if (metric != NBPG_METRIC_L2 && metric != NBPG_METRIC_MIPS)
+        return set_error(NBPG_EINVAL, "metric %d", metric);
+    ix->mips = metric == NBPG_METRIC_MIPS ? 1u : 0u;
     return NBPG_OK;
 }
 
@@ -450,7 +460,7 @@ int nbpg_l2_batch(const nbpg_index *ix, const void *d_query, const uint32_t *d_i
         uint32_t blocks = (uint32_t)((n + 255) / 256);
         hipLaunchKernelGGL((l2_batch_kernel<DT.value, NCH.value>), dim3(blocks), dim3(256), 0,
                            as_stream(stream), ix->vecs, ix->stride, ix->norms,
-                           (const uint8_t *)d_query, ix->dims, d_ids, n, ix->n, d_out);
+                           (const uint8_t *)d_query, ix->dims, d_ids, n, ix->n, d_out, ix->mips);
     });
     NBPG_TRY(rc);
     HIP_TRY(hipGetLastError());

@@ -28,6 +28,7 @@ struct nbpg_index {
     uint32_t *norms;
     uint32_t *nbr;
     uint32_t *deg;
+    uint32_t mips; // 0: squared L2, 1: inner product (negated dot)
     void *ws; // grow-only workspace for search/prune scratch
     size_t ws_bytes;
     // 128 u32 of device flags and counters, each user on its own 128-byte
@@ -69,7 +70,7 @@ int ensure_ws(nbpg_index *idx, size_t bytes);
 int device_repair(nbpg_index *ix, uint32_t sp, uint64_t *out_isolated);
 // rerank of candidate rows (rerank.hip); raw query rows at q_stride bytes
 #define RERANK_MAX 1024
-int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+int rerank_launch(int raw_dtype, uint32_t mips, const void *d_raw_base, uint64_t n, uint32_t dims,
                   const void *d_raw_queries, uint64_t q_stride, uint64_t nq,
                   const uint32_t *d_cand, uint32_t cand_stride, uint32_t n_cand, uint32_t k,
                   uint32_t rerank_factor, uint32_t *d_out_ids, float *d_out_dists,

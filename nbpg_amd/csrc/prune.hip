@@ -24,6 +24,7 @@ struct PruneArgs {
     const uint32_t *norms;
     uint64_t n;
     uint32_t stride;
+    uint32_t mips; // inner-product metric
     const uint32_t *nodes;
     uint64_t n_nodes;
     const nbpg_iddist *cand;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(64 * PRUNE_WAVES) void prune_kernel(PruneArgs a)
     const double alpha = a.alphas ? (double)a.alphas[t] : a.alpha;
     uint32_t comps = 0;
     const uint32_t sel = wave_prune<DT, NCH>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride,
-                                             a.norms, oid, od, comps, lane);
+                                             a.norms, oid, od, comps, lane, a.mips);
     if (lane == 0) {
         a.out_counts[t] = sel;
         if (a.out_comps)
@@ -126,6 +127,7 @@ extern "C" int nbpg_robust_prune_batch(const nbpg_index *ix, const uint32_t *d_n
     memset(&a, 0, sizeof(a));
     a.vecs = ix->vecs;
     a.norms = ix->norms;
+    a.mips = ix->mips;
     a.n = ix->n;
     a.stride = ix->stride;
     a.nodes = d_nodes;

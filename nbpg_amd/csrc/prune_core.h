@@ -82,7 +82,7 @@ template <int DT, int NCH>
 __device__ uint32_t wave_prune(uint64_t *keys, uint32_t m, uint32_t p, uint32_t R, double alpha,
                                double cosa, const uint8_t *__restrict__ vecs, uint32_t stride,
                                const uint32_t *__restrict__ norms, uint32_t *out_ids, float *out_d,
-                               uint32_t &comps, uint32_t lane)
+                               uint32_t &comps, uint32_t lane, uint32_t mips = 0)
 {
     const double eps = 1e-5;
     uint32_t sel = 0;
@@ -125,7 +125,7 @@ __device__ uint32_t wave_prune(uint64_t *keys, uint32_t m, uint32_t p, uint32_t 
             bool prune = false;
             if (live) {
                 const uint4 *r = reinterpret_cast<const uint4 *>(vecs + (uint64_t)pp * stride);
-                const float d_sp = dist_from<DT>(norms[pp], ns, dot_row<DT, NCH>(r, ss));
+                const float d_sp = dist_from<DT>(norms[pp], ns, dot_row<DT, NCH>(r, ss), mips);
                 const float d_pp = key_dist(kv);
                 // prune.h:227-238, alpha test in double
                 if (alpha > eps && alpha * (double)d_sp <= (double)d_pp)

@@ -188,6 +188,7 @@ struct GraphView {
     const uint32_t *norms;
     const uint32_t *nbr;
     uint32_t stride, R;
+    uint32_t mips; // inner-product metric
 };
 
 // One search.  `self` is skipped among neighbours (beamSearch.h:79, the
@@ -202,7 +203,7 @@ __device__ RefOut ref_search(RefState &s, const GraphView &g, const uint32_t (&q
     const uint32_t lim = min(R, dlim);
     auto dist_of = [&](uint32_t id) -> float {
         const uint4 *r = reinterpret_cast<const uint4 *>(g.vecs + (uint64_t)id * g.stride);
-        return dist_from<DT>(g.norms[id], nqn, dot_row<DT, NCH>(r, qv));
+        return dist_from<DT>(g.norms[id], nqn, dot_row<DT, NCH>(r, qv), g.mips);
     };
     uint64_t *Fp = s.F, *NFp = s.NF;
     for (uint32_t i = lane; i < H; i += WAVE)

@@ -75,7 +75,7 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t m)
 }
 
 // any row size or alignment: each lane streams whole rows from memory
-template <int RT>
+template <int RT, bool IP>
 __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
 {
     __shared__ uint64_t keys[4][RERANK_MAX];
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
         uint64_t key = ~0ull;
         if (i < m) {
             const uint32_t id = a.cand[q * a.cand_stride + i];
-            key = make_key(raw_row<RT>::dist(a.base + (uint64_t)id * a.row_bytes, qrow, a.dims), id);
+            key = make_key(raw_row<RT, IP>::dist(a.base + (uint64_t)id * a.row_bytes, qrow, a.dims), id);
         }
         kw[i] = key;
     }
@@ -125,7 +125,7 @@ __device__ __forceinline__ void rs_fetch(rs_vec (&buf)[8], rs_vec &qbuf, const u
         qbuf = *reinterpret_cast<const rs_vec *>(qrow + off);
 }
 
-template <int RT>
+template <int RT, bool IP>
 __global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32_t n2cap)
 {
     extern __shared__ __align__(16) uint8_t lds_rr[];
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32
         // the next step's pieces are loaded into registers while this
         // step's are summed (two steps in flight per wave)
         rs_vec buf[8], qbuf = {0, 0, 0, 0};
-        typename raw_acc<RT>::T acc = 0;
+        typename raw_acc<RT, IP>::T acc = 0;
         rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, 0, gm, piece, sub);
         for (uint64_t c0 = 0; c0 < a.row_bytes; c0 += RS_CH) {
             const bool in_row = c0 + 16u * piece < a.row_bytes;
@@ -159,19 +159,32 @@ __global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32
                 rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, c0 + RS_CH, gm, piece, sub);
             wave_sync();
             if (lane < gm)
-                acc = raw_acc<RT>::add(acc, stage + lane * RS_PITCH, stage + 64 * RS_PITCH,
+                acc = raw_acc<RT, IP>::add(acc, stage + lane * RS_PITCH, stage + 64 * RS_PITCH,
                                        (uint32_t)min<uint64_t>(RS_CH, a.row_bytes - c0));
             wave_sync();
         }
         if (lane < gm)
-            kw[g0 + lane] = make_key(raw_acc<RT>::fin(acc), my_id);
+            kw[g0 + lane] = make_key(raw_acc<RT, IP>::fin(acc), my_id);
     }
     for (uint32_t i = m + lane; i < n2; i += WAVE)
         kw[i] = ~0ull;
     sort_and_write(a, q, kw, m, n2, lane);
 }
 
-int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+template <typename F> static void rerank_dispatch(int raw_dtype, F &&go)
+{
+    using std::integral_constant;
+    if (raw_dtype == NBPG_F32)
+        go(integral_constant<int, NBPG_F32>());
+    else if (raw_dtype == NBPG_U8)
+        go(integral_constant<int, NBPG_U8>());
+    else if (raw_dtype == NBPG_I8)
+        go(integral_constant<int, NBPG_I8>());
+    else
+        go(integral_constant<int, NBPG_U4X2>()); // U4X2 and I4X2
+}
+
+int rerank_launch(int raw_dtype, uint32_t mips, const void *d_raw_base, uint64_t n, uint32_t dims,
                   const void *d_raw_queries, uint64_t q_stride, uint64_t nq,
                   const uint32_t *d_cand, uint32_t cand_stride, uint32_t n_cand, uint32_t k,
                   uint32_t rerank_factor, uint32_t *d_out_ids, float *d_out_dists,
@@ -209,22 +222,22 @@ int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t di
         const uint32_t per_wave = RS_STAGE + 8 * n2, wpg = 4 * per_wave <= 65536 ? 4 : 2;
         const dim3 sg((uint32_t)((nq + wpg - 1) / wpg)), sb(WAVE * wpg);
         const uint32_t lds = wpg * per_wave;
-        if (raw_dtype == NBPG_F32)
-            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_F32>, sg, sb, lds, s, a, n2);
-        else if (raw_dtype == NBPG_U8)
-            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_U8>, sg, sb, lds, s, a, n2);
-        else if (raw_dtype == NBPG_I8)
-            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_I8>, sg, sb, lds, s, a, n2);
-        else
-            hipLaunchKernelGGL(rerank_staged_kernel<NBPG_U4X2>, sg, sb, lds, s, a, n2);
-    } else if (raw_dtype == NBPG_F32)
-        hipLaunchKernelGGL(rerank_kernel<NBPG_F32>, grid, block, 0, s, a);
-    else if (raw_dtype == NBPG_U8)
-        hipLaunchKernelGGL(rerank_kernel<NBPG_U8>, grid, block, 0, s, a);
-    else if (raw_dtype == NBPG_I8)
-        hipLaunchKernelGGL(rerank_kernel<NBPG_I8>, grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL(rerank_kernel<NBPG_U4X2>, grid, block, 0, s, a);
+        auto go = [&](auto RT) {
+            if (mips)
+                hipLaunchKernelGGL((rerank_staged_kernel<RT.value, true>), sg, sb, lds, s, a, n2);
+            else
+                hipLaunchKernelGGL((rerank_staged_kernel<RT.value, false>), sg, sb, lds, s, a, n2);
+        };
+        rerank_dispatch(raw_dtype, go);
+    } else {
+        auto go = [&](auto RT) {
+            if (mips)
+                hipLaunchKernelGGL((rerank_kernel<RT.value, true>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((rerank_kernel<RT.value, false>), grid, block, 0, s, a);
+        };
+        rerank_dispatch(raw_dtype, go);
+    }
     return hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "rerank launch");
 }
 
@@ -232,7 +245,7 @@ int rerank_launch(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t di
 
 using namespace nbpg;
 
-extern "C" int nbpg_rerank(int raw_dtype, const void *d_raw_base, uint64_t n, uint32_t dims,
+extern "C" int nbpg_rerank(int raw_dtype, int metric, const void *d_raw_base, uint64_t n, uint32_t dims,
                            const void *d_raw_queries, uint64_t nq, const uint32_t *d_cand,
                            uint32_t cand_stride, uint32_t n_cand, uint32_t k,
                            uint32_t rerank_factor, uint32_t *d_out_ids, float *d_out_dists,
@@ -241,11 +254,14 @@ extern "C" int nbpg_rerank(int raw_dtype, const void *d_raw_base, uint64_t n, ui
     NBPG_TRY(ensure_init());
     if (raw_dtype < NBPG_U8 || raw_dtype > NBPG_F32)
         return set_error(NBPG_EINVAL, "raw dtype %d", raw_dtype);
+    if (metric != NBPG_METRIC_L2 && metric != NBPG_METRIC_MIPS)
+        return set_error(NBPG_EINVAL, "metric %d", metric);
     if (nq && (!d_raw_base || !d_raw_queries || !d_cand || !d_out_ids))
         return set_error(NBPG_EINVAL, "null argument");
     if (k == 0 || rerank_factor == 0 || n_cand == 0 || cand_stride < n_cand || dims == 0)
         return set_error(NBPG_EINVAL, "k, rerank_factor and n_cand must be positive");
-    return rerank_launch(raw_dtype, d_raw_base, n, dims, d_raw_queries,
+    return rerank_launch(raw_dtype, metric == NBPG_METRIC_MIPS ? 1u : 0u, d_raw_base, n, dims,
+                         d_raw_queries,
                          (uint64_t)dims * raw_elem_bytes(raw_dtype), nq, d_cand, cand_stride,
                          n_cand, k, rerank_factor, d_out_ids, d_out_dists, as_stream(stream));
 }

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
     extern __shared__ __align__(16) uint8_t lds_state[];
     uint8_t *st = LDS ? lds_state : a.gscratch + (uint64_t)blockIdx.x * a.state_bytes;
     RefState s = ref_state(st, a.L, a.R, a.maxv, a.H);
-    const GraphView g = {a.vecs, a.norms, a.nbr, a.stride, a.R};
+    const GraphView g = {a.vecs, a.norms, a.nbr, a.stride, a.R, a.mips};
     const uint32_t lane = threadIdx.x;
     for (uint64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
         uint4 qv[NCH];
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
                 for (uint32_t j = lane; j < n; j += WAVE) {
                     const uint32_t id = key_id(o.F[j]);
                     s.cand[j] = make_key(
-                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims), id);
+                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims, a.mips), id);
                 }
                 src = s.cand;
                 kk = min(a.k, n);
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
                 for (uint32_t j = lane; j < n; j += WAVE) {
                     const uint32_t id = key_id(s.vis[j]);
                     s.vis[j] = make_key(
-                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims), id);
+                        raw_dist(a.raw_dtype, a.raw_base + id * a.raw_row, rq, a.raw_dims, a.mips), id);
                 }
                 src = s.vis;
                 kk = min(a.k, min(o.fs, o.vs));
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
         {
             const uint32_t e = a.entry;
             const uint4 *r = reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)e * a.stride);
-            const float d = dist_from<DT>(a.norms[e], nqn, dot_row<DT, NCH>(r, qs));
+            const float d = dist_from<DT>(a.norms[e], nqn, dot_row<DT, NCH>(r, qs), a.mips);
             if (lane == 0) {
                 beam[0] = make_key(d, e);
                 filt[fast_slot(e, a.flog2)] = e;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
                 if (fresh[t]) {
                     const uint4 *r =
                         reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride);
-                    dd[t] = dist_from<DT>(a.norms[ids[t]], nqn, dot_row<DT, NCH>(r, qs));
+                    dd[t] = dist_from<DT>(a.norms[ids[t]], nqn, dot_row<DT, NCH>(r, qs), a.mips);
                 }
             }
             uint32_t nc = 0;
@@ -329,6 +329,7 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     a.qstride = qstride;
     a.nq = nq;
     a.qbytes = ix->dims;
+    a.mips = ix->mips;
     for (uint32_t s = 0; s < sp_num; ++s)
         a.sp[s] = sp[s];
     a.sp_num = sp_num;
@@ -406,6 +407,7 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     a.qstride = qstride;
     a.nq = nq;
     a.qbytes = ix->dims;
+    a.mips = ix->mips;
     a.L = L;
     a.k = k;
     a.Ls = d_L;
@@ -567,7 +569,7 @@ int nbpg_search_rerank(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
                          sp_num, qp->L, qp->L, nullptr, nullptr, qp->L, qp->L, qp->limit, fast,
                          cand, nullptr, d_out_stats, ix->counter, s));
-    return rerank_launch(raw_dtype, d_raw_base, ix->n, raw_dims, d_raw_queries, raw_query_stride,
+    return rerank_launch(raw_dtype, ix->mips, d_raw_base, ix->n, raw_dims, d_raw_queries, raw_query_stride,
                          nq, cand, qp->L, qp->L, qp->k, rerank_factor, d_out_ids, d_out_dists, s);
 }
 

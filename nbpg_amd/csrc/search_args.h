@@ -13,6 +13,7 @@ struct SearchArgs {
     const uint8_t *queries;
     uint64_t qstride, nq;
     uint32_t qbytes;
+    uint32_t mips; // inner-product metric
     // reference mode
     uint32_t sp[64]; // start points: reference mode (<= 32) or FAST seeds (<= 64)
     uint32_t sp_num;

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             if (lane < a.nseeds) {
                 key = make_key(dist_row_lds<DT, NCH>(
                                    reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)sid * a.stride),
-                                   ql, nqn),
+                                   ql, nqn, a.mips),
                                sid);
                 filt[fast_slot(sid, a.flog2)] = sid;
             }
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         } else {
             const uint32_t en = a.entry;
             const float d0 = dist_row_lds<DT, NCH>(
-                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), ql, nqn);
+                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), ql, nqn, a.mips);
             wave_sync();
             if (lane == 0) {
                 b[0] = make_key(d0, en);
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
                     key[t] = make_key(
                         dist_row_lds<DT, NCH>(
                             reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)ids[t] * a.stride),
-                            ql, nqn),
+                            ql, nqn, a.mips),
                         ids[t]);
                 cmps += (uint32_t)__popcll(ballot(fresh[t]));
             }
@@ -369,8 +369,10 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                     xq = dot_traits<DT>::dot(x[g].y, qc.y, xq);
                     xq = dot_traits<DT>::dot(x[g].z, qc.z, xq);
                     xq = dot_traits<DT>::dot(x[g].w, qc.w, xq);
-                    const uint32_t part = row_sum<LPR>(xx - 2u * xq);
-                    const float d = (float)(int32_t)((nqn + part) * dot_traits<DT>::scale);
+                    // inner product: the row's lanes sum <x_c, q_c> alone
+                    const uint32_t part = row_sum<LPR>(a.mips ? xq : xx - 2u * xq);
+                    const float d = a.mips ? -(float)(int32_t)(part * dot_traits<DT>::scale)
+                                           : (float)(int32_t)((nqn + part) * dot_traits<DT>::scale);
                     const uint64_t key = make_key(d, id[g]);
                     uint64_t m = ballot(c == 0 && r < nf && key < worst);
                     while (m) {

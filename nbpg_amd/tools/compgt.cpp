@@ -1,13 +1,14 @@
 // compgt.cpp -- CompGT's command line (CompGT.cpp:134-163) over nbpg_compgt.
 //
 //   CompGT --base_path B.u8bin|B.i8bin --query_path Q --gt_path OUT.gt [--k 100]
-//          [--engine mfma|plain] [--tie_rule reference|exact]
+//          [--engine mfma|plain] [--tie_rule reference|exact] [--use_mips]
 //
 // The element type follows the base file's extension as in the reference.
 // --tie_rule reference (the default) keeps the reference's slot-replacement
 // rule, so OUT.gt has the bytes the reference's CompGT writes; exact gives the
-// (dist, id) top-k.  Float rows (.fbin/.bin) and --use_mips are not on this
-// path and are refused.
+// (dist, id) top-k.  --use_mips ranks by the negated inner product
+// (CompGT.cpp:51).  Float rows (.fbin/.bin) are not on this path and are
+// refused.
 #include <chrono>
 #include <iostream>
 
@@ -23,8 +24,6 @@ int main(int argc, char **argv)
         const std::string bp = f.str("base_path"), qp = f.str("query_path"), gp = f.str("gt_path");
         if (bp.empty() || qp.empty() || gp.empty())
             throw std::runtime_error("--base_path, --query_path and --gt_path are required");
-        if (f.has("use_mips"))
-            throw std::runtime_error("--use_mips: inner-product ground truth is not on this path");
         int dtype;
         if (ends_with(bp, ".u8bin"))
             dtype = NBPG_U8;
@@ -48,6 +47,8 @@ int main(int argc, char **argv)
             mode |= NBPG_GT_REFTIES;
         else if (rule != "exact")
             throw std::runtime_error("--tie_rule reference|exact");
+        if (f.has("use_mips"))
+            mode |= NBPG_GT_MIPS;
         DevBuf db(base.data.data(), base.data.size()), dq(q.data.data(), q.data.size());
         DevBuf di((size_t)q.n * k * 4), dd((size_t)q.n * k * 4);
         const auto t0 = std::chrono::steady_clock::now();

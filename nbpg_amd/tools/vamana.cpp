@@ -22,7 +22,8 @@
 // the frontier head, 0 is the filtering mode (reference mode only); the raw
 // rows stay in device memory (--cache_raw is accepted and changes nothing).
 // 4-split builds draw their random start points from --sp_seed instead of
-// std::random_device.  Not on this path (refused): --use_mips,
+// std::random_device.  --use_mips builds and searches with the negated
+// inner product (DistanceMetric::MIPS).  Not on this path (refused):
 // --dir_bias_scale > 0.
 #include <algorithm>
 #include <chrono>
@@ -45,33 +46,41 @@ bool file_exists(const std::string &p)
     return stat(p.c_str(), &st) == 0;
 }
 
-// point.h:35-120 on the host, for the tie extension of the recall
-float host_dist(int dtype, const uint8_t *a, const uint8_t *b, uint32_t dims)
+// point.h:35-156 on the host, for the tie extension of the recall: squared
+// L2, or with mips the negated inner product
+float host_dist(int dtype, const uint8_t *a, const uint8_t *b, uint32_t dims, bool mips)
 {
-    if (dtype == NBPG_F32) { // point.h:137-146, summed in index order
+    if (dtype == NBPG_F32) { // point.h:137-156, summed in index order
         const float *p = reinterpret_cast<const float *>(a), *q = reinterpret_cast<const float *>(b);
         float r = 0.0f;
         for (uint32_t i = 0; i < dims; ++i) {
-            const float d = p[i] - q[i];
-            r += d * d;
+            if (mips) {
+                r += p[i] * q[i];
+            } else {
+                const float d = p[i] - q[i];
+                r += d * d;
+            }
         }
-        return r;
+        return mips ? -r : r;
     }
-    int64_t s = 0;
+    int32_t s = 0;
     for (uint32_t i = 0; i < dims; ++i) {
+        int xa, xb, ya = 0, yb = 0; // the element pair, or the two nibble pairs
         if (dtype == NBPG_U8) {
-            const int d = (int)a[i] - (int)b[i];
-            s += d * d;
+            xa = a[i];
+            xb = b[i];
         } else if (dtype == NBPG_I8) {
-            const int d = (int)(int8_t)a[i] - (int)(int8_t)b[i];
-            s += d * d;
+            xa = (int8_t)a[i];
+            xb = (int8_t)b[i];
         } else {
-            const int dh = (int)(a[i] & 0xF0) - (int)(b[i] & 0xF0);
-            const int dl = (int)((a[i] << 4) & 0xFF) - (int)((b[i] << 4) & 0xFF);
-            s += dh * dh + dl * dl;
+            xa = a[i] & 0xF0;
+            xb = b[i] & 0xF0;
+            ya = (a[i] << 4) & 0xFF;
+            yb = (b[i] << 4) & 0xFF;
         }
+        s += mips ? xa * xb + ya * yb : (xa - xb) * (xa - xb) + (ya - yb) * (ya - yb);
     }
-    return (float)s;
+    return mips ? -(float)s : (float)s;
 }
 
 struct Result {
@@ -130,8 +139,7 @@ int main(int argc, char **argv)
         const std::string bp = f.str("quant_base_path"), ip = f.str("index_path");
         if (bp.empty() || ip.empty())
             throw std::runtime_error("--quant_base_path and --index_path are required");
-        if (f.has("use_mips"))
-            throw std::runtime_error("--use_mips: inner-product graphs are not on this path");
+        const bool mips = f.has("use_mips");
         if (f.num("dir_bias_scale", 0) > 0)
             throw std::runtime_error("--dir_bias_scale > 0 is not on this path");
         int dtype;
@@ -178,10 +186,12 @@ int main(int argc, char **argv)
             if (n != base.n)
                 throw std::runtime_error("graph and base sizes differ");
             nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, gR), "nbpg_index_create");
+            nbpg_ok(nbpg_index_set_metric(ix, mips ? NBPG_METRIC_MIPS : NBPG_METRIC_L2), "set_metric");
             nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
             nbpg_ok(nbpg_index_load_graph(ix, gfile.c_str()), "load_graph");
         } else {
             nbpg_ok(nbpg_index_create(&ix, dtype, n, base.dims, R), "nbpg_index_create");
+            nbpg_ok(nbpg_index_set_metric(ix, mips ? NBPG_METRIC_MIPS : NBPG_METRIC_L2), "set_metric");
             nbpg_ok(nbpg_index_set_points(ix, base.data.data(), base.row_bytes(), 0, n), "set_points");
             if (!f.str("pre_index_path").empty()) { // Vamana.cpp:174-177
                 const std::string pf = f.str("pre_index_path") + ".graph";
@@ -212,7 +222,7 @@ int main(int argc, char **argv)
         const double avg_deg = (double)sum / (double)n;
         std::ostringstream params;
         params << "R = " << gR << ", L = " << p.L << ", alpha = (" << std::fixed << std::setprecision(2)
-               << p.min_alpha << "," << p.max_alpha << "), use_mips = 0";
+               << p.min_alpha << "," << p.max_alpha << "), use_mips = " << (mips ? 1 : 0);
         std::cout << "Vamana graph built with " << n << " points and parameters " << params.str()
                   << std::endl;
         std::cout << "Graph has average degree " << avg_deg << ", popular average degree 0"
@@ -307,7 +317,7 @@ int main(int argc, char **argv)
                 const uint8_t *qrow = rq.data.data() + i * rq.row_bytes();
                 auto dist = [&](uint32_t id) {
                     return host_dist(xdtype, rb.data.data() + (uint64_t)id * rb.row_bytes(), qrow,
-                                     rb.dims);
+                                     rb.dims, mips);
                 };
                 std::vector<uint32_t> with_ties(gt.begin() + i * gk, gt.begin() + i * gk + k);
                 const float last = dist(gt[i * gk + k - 1]);

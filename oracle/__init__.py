@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 U8, I8, U4X2, I4X2, F32 = 0, 1, 2, 3, 4
+MIPS = 0x100  # OR into a dtype: the inner-product metric
 
 _lib = None
 _ref = None
@@ -172,7 +173,7 @@ def _p(a):
 def points(arr, dtype):
     """Points view over a C-contiguous 2-D array (rows are packed bytes)."""
     a = np.ascontiguousarray(arr)
-    esz = 4 if dtype == F32 else 1
+    esz = 4 if (dtype & 0xFF) == F32 else 1
     P = Points(a.ctypes.data, a.shape[1] * esz, a.shape[0], a.shape[1], dtype)
     P._keep = a
     return P

@@ -65,8 +65,40 @@ static float l2_f32(const float *p, const float *q, uint32_t d)
     return s;
 }
 
+/* mips_distance (point.h:46-156): the negated inner product, integer types
+ * summed exactly in int32, f32 in index order (nibbles as above) */
+static float ip_int(int dtype, const uint8_t *p, const uint8_t *q, uint32_t d)
+{
+    int32_t s = 0;
+    for (uint32_t i = 0; i < d; ++i) {
+        if (dtype == NBPGO_U8)
+            s += (int32_t)p[i] * (int32_t)q[i];
+        else if (dtype == NBPGO_I8)
+            s += (int32_t)(int8_t)p[i] * (int32_t)(int8_t)q[i];
+        else {
+            s += (int32_t)(p[i] & 0xF0) * (int32_t)(q[i] & 0xF0);
+            s += (int32_t)(uint8_t)(p[i] << 4) * (int32_t)(uint8_t)(q[i] << 4);
+        }
+    }
+    return -(float)s;
+}
+
+static float ip_f32(const float *p, const float *q, uint32_t d)
+{
+    float s = 0.0f;
+    for (uint32_t i = 0; i < d; ++i)
+        s += p[i] * q[i];
+    return -s;
+}
+
 float nbpgo_distance(int dtype, const void *a, const void *b, uint32_t dims)
 {
+    if (dtype & NBPGO_MIPS) {
+        dtype &= ~NBPGO_MIPS;
+        if (dtype == NBPGO_F32)
+            return ip_f32((const float *)a, (const float *)b, dims);
+        return ip_int(dtype, (const uint8_t *)a, (const uint8_t *)b, dims);
+    }
     switch (dtype) {
     case NBPGO_U8: return l2_u8((const uint8_t *)a, (const uint8_t *)b, dims);
     case NBPGO_I8: return l2_i8((const int8_t *)a, (const int8_t *)b, dims);
@@ -136,7 +168,7 @@ static int cmp_by_id(const void *x, const void *y)
 /* CompGT: CompGT.cpp:29-134                                            */
 /* ------------------------------------------------------------------ */
 
-static size_t elem_size(int dtype) { return dtype == NBPGO_F32 ? 4 : 1; }
+static size_t elem_size(int dtype) { return (dtype & ~NBPGO_MIPS) == NBPGO_F32 ? 4 : 1; }
 
 int nbpgo_compgt(int dtype, const void *base, uint64_t n, const void *queries, uint64_t nq,
                  uint32_t dims, uint32_t k, int tie_mode, uint32_t *out_ids, float *out_dists,

@@ -23,6 +23,8 @@ extern "C" {
 /* element types, selected by file extension in the reference
  * (Vamana.cpp:292-311, CompGT.cpp:152-163) */
 enum { NBPGO_U8 = 0, NBPGO_I8 = 1, NBPGO_U4X2 = 2, NBPGO_I4X2 = 3, NBPGO_F32 = 4 };
+/* OR into a dtype: the inner-product metric (DistanceMetric::MIPS, point.h:20-24) */
+enum { NBPGO_MIPS = 0x100 };
 
 /* (id, dist) record; order = dist asc, id asc (query_params.h:97-119) */
 typedef struct {

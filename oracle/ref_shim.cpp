@@ -31,11 +31,21 @@ using namespace ant;
 
 namespace {
 
+// the metric of the current call: a dtype argument | 0x100 selects the
+// inner product (DistanceMetric::MIPS) for every range the call builds
+DistanceMetric g_metric = DistanceMetric::EUCLIDEAN;
+
+int take_metric(int dtype)
+{
+    g_metric = (dtype & 0x100) ? DistanceMetric::MIPS : DistanceMetric::EUCLIDEAN;
+    return dtype & 0xFF;
+}
+
 template <typename T>
 PointRange<Point<T>> make_points(const uint8_t *data, uint64_t n, uint32_t dims)
 {
     using PR = PointRange<Point<T>>;
-    PR P(n, typename PR::Parameters(dims));
+    PR P(n, typename PR::Parameters(dims, g_metric));
     const size_t nb = P.params.num_bytes();
     for (uint64_t i = 0; i < n; ++i)
         std::memcpy(reinterpret_cast<uint8_t *>(P[i].data()), data + i * nb, nb);
@@ -53,6 +63,8 @@ Graph<uint32_t> make_graph(const uint32_t *rows, uint64_t n, uint32_t R)
 template <typename T>
 float dist_t(const void *a, const void *b, uint32_t dims)
 {
+    if (g_metric == DistanceMetric::MIPS)
+        return mips_distance(reinterpret_cast<const T *>(a), reinterpret_cast<const T *>(b), dims);
     return euclidean_distance(reinterpret_cast<const T *>(a), reinterpret_cast<const T *>(b), dims);
 }
 
@@ -174,6 +186,7 @@ extern "C" {
 
 float nbpgr_distance(int dtype, const void *a, const void *b, uint32_t dims)
 {
+    dtype = take_metric(dtype);
     switch (dtype) {
     case 0: return dist_t<uint8_t>(a, b, dims);
     case 1: return dist_t<int8_t>(a, b, dims);
@@ -196,6 +209,7 @@ int nbpgr_search(int dtype, const uint8_t *base, uint64_t n, uint32_t dims, cons
                  double cut, uint64_t limit, uint32_t degree_limit, uint32_t entry,
                  uint32_t *out_ids, uint32_t *out_stats)
 {
+    dtype = take_metric(dtype);
     switch (dtype) {
     case 0: return search_t<uint8_t>(base, n, dims, rows, R, queries, nq, L, k, cut, limit, degree_limit, entry, out_ids, out_stats);
     case 1: return search_t<int8_t>(base, n, dims, rows, R, queries, nq, L, k, cut, limit, degree_limit, entry, out_ids, out_stats);
@@ -212,6 +226,7 @@ int nbpgr_search_exact(int dtype, const uint8_t *base, uint64_t n, uint32_t dims
                        uint64_t limit, uint32_t degree_limit, uint32_t entry,
                        uint32_t rerank_factor, uint32_t *out_ids, uint32_t *out_stats)
 {
+    dtype = take_metric(dtype);
     switch (dtype) {
     case 0: return search_exact_x<uint8_t>(raw_dtype, base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats);
     case 2: return search_exact_x<uint4_2>(raw_dtype, base, n, dims, rows, R, queries, nq, raw_base, raw_dims, raw_queries, L, k, cut, limit, degree_limit, entry, rerank_factor, out_ids, out_stats);
@@ -232,6 +247,7 @@ uint64_t nbpgr_repair(uint32_t *rows, uint64_t n, uint32_t R, uint32_t sp)
 uint64_t nbpgr_prune(int dtype, const uint8_t *base, uint64_t n, uint32_t dims, uint32_t p,
                      void *cand, uint64_t size, uint32_t R, double alpha, double cos_angle)
 {
+    dtype = take_metric(dtype);
     auto *c = reinterpret_cast<IdDist<uint32_t> *>(cand);
     switch (dtype) {
     case 0: return prune_t<uint8_t>(base, n, dims, p, c, size, R, alpha, cos_angle);
@@ -246,6 +262,7 @@ double nbpgr_prune_batch(int dtype, const uint8_t *base, uint64_t n, uint32_t di
                          const uint32_t *nodes, uint64_t P, void *cand, uint32_t C, uint32_t R,
                          double alpha, double cos_angle, uint32_t *out_counts)
 {
+    dtype = take_metric(dtype);
     auto *c = reinterpret_cast<IdDist<uint32_t> *>(cand);
     switch (dtype) {
     case 0: return prune_batch_t<uint8_t>(base, n, dims, nodes, P, c, C, R, alpha, cos_angle, out_counts);

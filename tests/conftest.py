@@ -38,6 +38,14 @@ def golden_repair():
 
 
 @pytest.fixture(scope="session")
+def golden_mips():
+    """Inner-product (--use_mips) outputs recorded from the compiled reference."""
+    path = os.path.join(ROOT, "tests", "golden", "reference_mips.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
 def gpu():
     """Initialised nbpg_amd on device 0 (GPU tests only)."""
     import torch

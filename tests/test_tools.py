@@ -59,8 +59,9 @@ def test_unknown_and_missing_flags_are_errors(name):
 def test_refused_options():
     r = run([tool("CompGT"), "--base_path", "b.fbin", "--query_path", "q.fbin", "--gt_path", "g.gt"])
     assert r.returncode == 1 and "float rows" in r.stderr
-    r = run([tool("Vamana"), "--quant_base_path", "b.u8bin", "--index_path", "g", "--use_mips"])
-    assert r.returncode == 1 and "use_mips" in r.stderr
+    r = run([tool("Vamana"), "--quant_base_path", "b.u8bin", "--index_path", "g",
+             "--dir_bias_scale", "0.5"])
+    assert r.returncode == 1 and "dir_bias_scale" in r.stderr
 
 
 @pytest.mark.gpu
@@ -155,6 +156,42 @@ def test_vamana_tool_rerank_sweep_matches_reference_binary(gpu, golden_exact, tm
     a, b = sweep_lines(ours.stdout), sweep_lines(ref.stdout)
     assert len(a) == len(b) > 60
     assert a == b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ext", [("u8", "u8bin"), ("i8", "i8bin")])
+def test_mips_tools_match_reference_binaries(gpu, golden_mips, tmp_path, name, ext):
+    """--use_mips: CompGT writes the reference binary's .gt, Vamana builds its
+    graph, and every point of the recall sweep equals the reference's."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "Vamana")
+    if not os.path.exists(ref_bin):
+        pytest.skip("oracle/_ref not built")
+    z = golden_mips
+    b, q = tmp_path / ("b." + ext), tmp_path / ("q." + ext)
+    write_bin(b, z[name + "_base"])
+    write_bin(q, z[name + "_queries"])
+    gt = tmp_path / "g.gt"
+    r = run([tool("CompGT"), "--base_path", str(b), "--query_path", str(q), "--gt_path", str(gt),
+             "--k", "50", "--use_mips"])
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(gt, dtype=np.uint32)
+    nq, k = int(raw[0]), int(raw[1])
+    assert (raw[2:2 + nq * k].reshape(nq, k) == z[name + "_gt_ids"]).all()
+    assert (raw[2 + nq * k:].view(np.float32).reshape(nq, k) == z[name + "_gt_dists"]).all()
+    build = ["--quant_base_path", str(b), "--R", "24", "--L", "48", "--max_alpha", "1.2",
+             "--num_split", "0", "--num_passes", "1", "--use_mips"]
+    r = run([tool("Vamana"), "--index_path", str(tmp_path / "g"), "--rebuild"] + build)
+    assert r.returncode == 0, r.stderr
+    assert (live(read_graph(tmp_path / "g.graph")) == live(z[name + "_graph1"])).all()
+    sweep = build + ["--index_path", str(tmp_path / "g"), "--quant_query_path", str(q),
+                     "--gt_path", str(gt), "--k", "10", "--verbose"]
+    ours = run([tool("Vamana")] + sweep + ["--res_path", str(tmp_path / "ours.csv")])
+    assert ours.returncode == 0, ours.stderr
+    ref = subprocess.run([ref_bin] + sweep + ["--res_path", str(tmp_path / "ref.csv")],
+                         capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    a, c = sweep_lines(ours.stdout), sweep_lines(ref.stdout)
+    assert len(a) == len(c) > 60 and a == c
 
 
 @pytest.mark.gpu
