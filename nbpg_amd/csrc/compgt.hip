@@ -209,9 +209,25 @@ __global__ __launch_bounds__(256) void compgt_merge_kernel(const uint64_t *__res
     }
 }
 
+int compgt_merge_launch(const uint64_t *partial, uint64_t nq, uint32_t slices, uint32_t k,
+                        uint32_t *ids, float *dists, hipStream_t s)
+{
+    uint32_t m = slices * k, n2 = 1;
+    while (n2 < m)
+        n2 <<= 1;
+    hipLaunchKernelGGL(compgt_merge_kernel, dim3((uint32_t)nq), dim3(256), n2 * 8, s, partial,
+                       slices, k, ids, dists);
+    return hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "compgt merge launch");
+}
+
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
                 uint32_t dims, uint32_t k, uint32_t mips, uint32_t *out_ids, float *out_dists,
                 hipStream_t stream);
+
+// exact top-k from distance tiles; base row r at base + r * row_stride
+int compgt_tiles(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                 const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k, bool use_mfma,
+                 uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
 
 int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
                    uint32_t dims, uint32_t k, bool use_mfma, uint32_t mips, uint32_t *ids,
@@ -252,13 +268,7 @@ int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row
     if (rc == NBPG_OK && hipGetLastError() != hipSuccess)
         rc = set_error(NBPG_EHIP, "compgt launch");
     if (rc == NBPG_OK) {
-        uint32_t m = (uint32_t)(slices * k), n2 = 1;
-        while (n2 < m)
-            n2 <<= 1;
-        hipLaunchKernelGGL(compgt_merge_kernel, dim3((uint32_t)nq), dim3(256), n2 * 8, s, partial,
-                           (uint32_t)slices, k, ids, dists);
-        if (hipGetLastError() != hipSuccess)
-            rc = set_error(NBPG_EHIP, "compgt merge launch");
+        rc = compgt_merge_launch(partial, nq, (uint32_t)slices, k, ids, dists, s);
     }
     hipFreeAsync(partial, s);
     return rc;
@@ -301,9 +311,13 @@ extern "C" int nbpg_compgt(int dtype, const void *d_base, uint64_t n, const void
         return compgt_refrule(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq,
                               dims, k, engine == NBPG_GT_MFMA, mips, d_out_ids, d_out_dists, s);
     }
-    if (mode == NBPG_GT_MFMA)
+    if (mode == NBPG_GT_MFMA) {
+        if ((dtype != NBPG_U8 && dtype != NBPG_I8) || dims % 16 ||
+            ((reinterpret_cast<uintptr_t>(d_base) | reinterpret_cast<uintptr_t>(d_queries)) & 15))
+            return set_error(NBPG_EUNSUP, "MFMA ground truth needs 16-byte u8 / i8 rows");
         return compgt_mfma(dtype, (const uint8_t *)d_base, n, (const uint8_t *)d_queries, nq, dims,
                            k, mips, d_out_ids, d_out_dists, s);
+    }
     if (mode == NBPG_GT_PLAIN)
         return compgt_plain_launch(dtype, (const uint8_t *)d_base, n, dims,
                                    (const uint8_t *)d_queries, nq, dims, k, mips, d_out_ids,

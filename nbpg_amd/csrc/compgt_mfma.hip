@@ -14,6 +14,7 @@
 //   3. select: per query, a bitonic sort of its candidates; a query whose
 //      list overflowed is recomputed by the plain kernel.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "internal.h"
@@ -23,12 +24,15 @@ namespace nbpg {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 #define MF_CAND 4096 // candidate slots per query
-#define MF_QW 2      // 16-query A tiles per wave
+#define MF_QW 4      // 16-query A tiles per wave
 #define MF_WAVES 4
 
 int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
                         const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
                         uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
+int compgt_tiles(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                 const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k, bool use_mfma,
+                 uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
 
 // per row, with x' = x ^ 0x80 = x - 128 (u8) or x (i8), over dims bytes:
 // L2: |x'|^2.  Inner product: half of what turns <x',q'> back into <x,q>,
@@ -104,7 +108,11 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
     const uint32_t xmask = is_u8 ? 0x80808080u : 0u;
     // query fragments: lane supplies 16 bytes of query row (l % 16) at k-chunk (l / 16)
     v4i a[MF_QW][KS];
-    uint64_t th[MF_QW][4]; // 0 = no query (keys are compared with <)
+    // the admission test in integers: with T the threshold distance (an
+    // exact integer), L2 admits xn - 2c <= T - qn, the inner product
+    // c + xn >= -T - qn; U holds the right-hand sides (never true for a
+    // missing query)
+    int32_t U[MF_QW][4];
     uint32_t qn[MF_QW][4];
 #pragma unroll
     for (int t = 0; t < MF_QW; ++t) {
@@ -122,16 +130,20 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const uint64_t qr = q0 + 16 * t + 4 * kq + r;
-            th[t][r] = qr < nq ? thr[qr] + 1 : 0ull; // thr < ~0: finite distance
             qn[t][r] = qr < nq ? nqv[qr] : 0u;
+            const int32_t T = qr < nq ? (int32_t)key_dist(thr[qr]) : 0;
+            U[t][r] = qr >= nq ? (mips ? INT_MAX : INT_MIN)
+                               : (mips ? -T - (int32_t)qn[t][r] : T - (int32_t)qn[t][r]);
         }
     }
     const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows;
     const uint64_t r1 = min(n, r0 + slice_rows);
-    for (uint64_t b0 = r0; b0 < r1; b0 += 16) {
+    // the next 16-row block's operands are loaded while this block is scored
+    v4i bn[KS];
+    uint32_t xnn = 0;
+    auto fetch = [&](uint64_t b0) {
         const uint64_t row = b0 + row16;
         const bool rv = row < r1;
-        v4i b[KS];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const uint32_t off = 64 * s + 16 * kq;
@@ -140,9 +152,23 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
                 const uint4 u = *reinterpret_cast<const uint4 *>(base + row * dims + off);
                 v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
             }
-            b[s] = v;
+            bn[s] = v;
         }
-        const uint32_t xn = rv ? nx[row] : 0u;
+        xnn = rv ? nx[row] : 0u;
+    };
+    fetch(r0);
+    for (uint64_t b0 = r0; b0 < r1; b0 += 16) {
+        const uint64_t row = b0 + row16;
+        const bool rv = row < r1;
+        v4i b[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            b[s] = bn[s];
+        const uint32_t xn = xnn;
+        if (b0 + 16 < r1)
+            fetch(b0 + 16);
+        // a missing row fails every test
+        const int32_t xs = rv ? (int32_t)xn : (mips ? -(1 << 30) : (1 << 30));
 #pragma unroll
         for (int t = 0; t < MF_QW; ++t) {
             v4i c = {0, 0, 0, 0};
@@ -151,14 +177,14 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
                 c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
-                                     : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
-                const uint64_t key = make_key(d, (uint32_t)row);
-                if (rv && key < th[t][r]) {
+                const bool adm = mips ? c[r] + xs >= U[t][r] : xs - 2 * c[r] <= U[t][r];
+                if (adm) {
+                    const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
+                                         : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
                     const uint64_t qr = q0 + 16 * t + 4 * kq + r;
                     const uint32_t p = atomicAdd(&cnt[qr], 1u);
                     if (p < MF_CAND)
-                        cand[qr * MF_CAND + p] = key;
+                        cand[qr * MF_CAND + p] = make_key(d, (uint32_t)row);
                 }
             }
         }
@@ -249,7 +275,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         cleanup();
         return set_error(NBPG_ENOMEM, "MFMA ground-truth workspace");
     }
-    int rc = compgt_plain_launch(dtype, base, S, step * dims, queries, nq, dims, k, mips, tid, td, s);
+    int rc = compgt_tiles(dtype, base, S, step * dims, queries, nq, dims, k, true, mips, tid, td, s);
     if (rc != NBPG_OK) {
         cleanup();
         return rc;

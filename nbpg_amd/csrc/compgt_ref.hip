@@ -12,6 +12,12 @@
 // positions.  This kernel replays that sequence row by row in id order and
 // sorts the k slots by (dist, id) at the end (CompGT.cpp:102-118).
 //
+// The same distance tiles also drive the exact (dist, id) top-k
+// (compgt_tile_topk_kernel): every query keeps a sorted list of its k
+// smallest keys in LDS and a row enters it only when its distance is at most
+// the list's k-th (one float compare and a ballot per 64 rows); the base is
+// cut into slices whose lists are merged afterwards.
+//
 // A workgroup owns RQ queries and walks the whole base in 256-row chunks:
 // its four wavefronts first compute the RQ x 256 distance tile (int8 MFMA
 // Gram tiles for 16-byte u8 / i8 rows, packed dot instructions otherwise),
@@ -23,6 +29,10 @@
 #include "internal.h"
 
 namespace nbpg {
+
+// compgt.hip: sorts each query's slices x k keys and writes the first k
+int compgt_merge_launch(const uint64_t *partial, uint64_t nq, uint32_t slices, uint32_t k,
+                        uint32_t *ids, float *dists, hipStream_t s);
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -37,7 +47,8 @@ struct RefGtArgs {
     uint32_t dims, k;
     uint32_t *ids;
     float *dists;
-    uint32_t mips; // inner-product metric
+    uint32_t mips;       // inner-product metric
+    uint64_t row_stride; // bytes between base rows (dims, or a multiple to sample)
 };
 
 __device__ __forceinline__ uint64_t wave_max64(uint64_t v)
@@ -153,7 +164,7 @@ template <int KS, int QT> struct MfmaTile {
                 const uint32_t off = 64 * s + 16 * kq;
                 v4i v = {0, 0, 0, 0};
                 if (rv && off < g.dims) {
-                    const uint4 u = *reinterpret_cast<const uint4 *>(g.base + row * g.dims + off);
+                    const uint4 u = *reinterpret_cast<const uint4 *>(g.base + row * g.row_stride + off);
                     v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask),
                             (int)(u.w ^ xmask)};
                 }
@@ -218,7 +229,7 @@ template <int DT, int NCH, int QT> struct DotTile {
         const bool rv = row < g.n;
         const uint32_t words = g.dims / 4;
         uint32_t x[NW];
-        const uint32_t *rw = reinterpret_cast<const uint32_t *>(g.base + (rv ? row : 0) * g.dims);
+        const uint32_t *rw = reinterpret_cast<const uint32_t *>(g.base + (rv ? row : 0) * g.row_stride);
 #pragma unroll
         for (int w = 0; w < NW; ++w)
             x[w] = (rv && (uint32_t)w < words) ? rw[w] : 0u;
@@ -333,6 +344,176 @@ __global__ __launch_bounds__(RG_THREADS) void compgt_refrule_kernel(RefGtArgs g,
     }
 }
 
+// exact top-k over base rows [r0, r1) of slice blockIdx.y: RQ queries per
+// workgroup; per query a sorted key list top[q][0:cnt) in LDS
+template <typename Tile>
+__global__ __launch_bounds__(RG_THREADS) void compgt_tile_topk_kernel(RefGtArgs g, uint32_t is_u8,
+                                                                      uint64_t slice_rows,
+                                                                      uint32_t slices,
+                                                                      uint64_t *partial)
+{
+    constexpr int RQ = Tile::RQ;
+    extern __shared__ __align__(16) uint8_t lds[];
+    float *D = reinterpret_cast<float *>(lds);
+    uint64_t *top = reinterpret_cast<uint64_t *>(D + RQ * RG_TS);
+    uint32_t *tn = reinterpret_cast<uint32_t *>(top + (size_t)RQ * g.k);
+    void *extra = tn + RQ;
+    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const uint64_t q0 = (uint64_t)blockIdx.x * RQ;
+    const uint32_t k = g.k;
+    const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows;
+    const uint64_t r1 = min(g.n, r0 + slice_rows);
+    Tile tile;
+    tile.init(g, q0, is_u8, extra, lane);
+    if (threadIdx.x < RQ)
+        tn[threadIdx.x] = 0;
+    for (uint64_t c0 = r0; c0 < r1; c0 += RG_CH) {
+        __syncthreads(); // the previous chunk's lists are done with D
+        tile.fill(g, c0, D, wv, lane);
+        __syncthreads();
+        for (uint32_t j = 0; j < RQ / 4; ++j) {
+            const uint32_t q = wv * (RQ / 4) + j;
+            if (q0 + q >= g.nq)
+                break;
+            uint64_t *tq = top + (size_t)q * k;
+            uint32_t cnt = tn[q];
+            uint64_t thr = cnt < k ? ~0ull : tq[k - 1];
+            float thr_d = cnt < k ? __builtin_huge_valf() : key_dist(thr);
+            for (uint32_t gr = 0; gr < RG_CH / WAVE; ++gr) {
+                const uint64_t row = c0 + gr * WAVE + lane;
+                const bool rv = row < r1;
+                const float v = D[q * RG_TS + gr * WAVE + lane];
+                // a superset by distance, then the exact (dist, id) test
+                uint64_t m = ballot(rv && v <= thr_d);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint64_t key = make_key(__shfl(v, l), (uint32_t)(c0 + gr * WAVE + l));
+                    if (key >= thr)
+                        continue;
+                    // insert at pos = #entries below key; the list grows to k
+                    uint32_t c = 0;
+                    for (uint32_t i = lane; i < cnt; i += WAVE)
+                        c += tq[i] < key;
+                    const uint32_t pos = wave_sum(c);
+                    const uint32_t ncnt = min(cnt + 1, k);
+                    for (int32_t t = (int32_t)ncnt - 2; t >= (int32_t)pos; t -= WAVE) {
+                        const int32_t idx = t - (int32_t)lane;
+                        uint64_t x = 0;
+                        if (idx >= (int32_t)pos)
+                            x = tq[idx];
+                        wave_sync();
+                        if (idx >= (int32_t)pos)
+                            tq[idx + 1] = x;
+                        wave_sync();
+                    }
+                    if (lane == 0)
+                        tq[pos] = key;
+                    wave_sync();
+                    cnt = ncnt;
+                    if (cnt == k) {
+                        thr = tq[k - 1];
+                        thr_d = key_dist(thr);
+                    }
+                }
+            }
+            if (lane == 0)
+                tn[q] = cnt;
+            wave_sync();
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = wv; q < RQ; q += RG_THREADS / WAVE) {
+        if (q0 + q >= g.nq)
+            break;
+        const uint32_t cnt = tn[q];
+        for (uint32_t i = lane; i < k; i += WAVE)
+            partial[((q0 + q) * slices + blockIdx.y) * k + i] =
+                i < cnt ? top[(size_t)q * k + i] : ~0ull;
+    }
+}
+
+template <int RQ> static size_t tile_topk_lds(uint32_t k, size_t extra)
+{
+    return (size_t)RQ * RG_TS * 4 + (size_t)RQ * k * 8 + RQ * 4 + extra;
+}
+
+template <typename Tile>
+static int launch_tile_topk(const RefGtArgs &g, uint32_t is_u8, size_t extra, hipStream_t s)
+{
+    constexpr int RQ = Tile::RQ;
+    const size_t lds = tile_topk_lds<RQ>(g.k, extra);
+    if (lds > (size_t)g_dev.lds_max)
+        return set_error(NBPG_EUNSUP, "tile ground truth: k = %u too large", g.k);
+    // slices of whole 256-row chunks: enough workgroups to fill the device,
+    // at most 8192 merged keys per query
+    const uint64_t groups = (g.nq + RQ - 1) / RQ;
+    uint64_t slices = std::max<uint64_t>(1, ((uint64_t)g_dev.cu * 4 + groups - 1) / groups);
+    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, 8192 / g.k));
+    slices = std::min<uint64_t>(slices, (g.n + RG_CH - 1) / RG_CH);
+    const uint64_t chunks = (g.n + RG_CH - 1) / RG_CH;
+    const uint64_t slice_rows = (chunks + slices - 1) / slices * RG_CH;
+    slices = (g.n + slice_rows - 1) / slice_rows;
+    uint64_t *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, g.nq * slices * g.k * 8, s) != hipSuccess)
+        return set_error(NBPG_ENOMEM, "tile ground truth partial lists");
+    hipLaunchKernelGGL((compgt_tile_topk_kernel<Tile>), dim3((uint32_t)groups, (uint32_t)slices),
+                       dim3(RG_THREADS), lds, s, g, is_u8, slice_rows, (uint32_t)slices, partial);
+    int rc = hipGetLastError() == hipSuccess ? NBPG_OK
+                                             : set_error(NBPG_EHIP, "tile ground truth launch");
+    if (rc == NBPG_OK)
+        rc = compgt_merge_launch(partial, g.nq, (uint32_t)slices, g.k, g.ids, g.dists, s);
+    hipFreeAsync(partial, s);
+    return rc;
+}
+
+// the exact (dist, id) top-k from distance tiles (MFMA tiles when use_mfma)
+int compgt_tiles(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                 const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k, bool use_mfma,
+                 uint32_t mips, uint32_t *ids, float *dists, hipStream_t s)
+{
+    RefGtArgs g;
+    g.row_stride = row_stride;
+    g.base = base;
+    g.queries = queries;
+    g.n = n;
+    g.nq = nq;
+    g.dims = dims;
+    g.k = k;
+    g.ids = ids;
+    g.dists = dists;
+    g.mips = mips;
+    if (use_mfma) {
+        // the most queries per workgroup whose tile and lists fit the LDS:
+        // every query group streams the whole base once
+        const uint32_t is_u8 = dtype == NBPG_U8;
+        // ... while at least three workgroups still share a CU's LDS
+        const size_t budget = (size_t)g_dev.lds_max / 3;
+        const int qt = tile_topk_lds<64>(k, 0) <= budget ? 4 : (tile_topk_lds<32>(k, 0) <= budget ? 2 : 1);
+        auto go = [&](auto KS) -> int {
+            constexpr int ks = decltype(KS)::value;
+            if (qt == 4)
+                return launch_tile_topk<MfmaTile<ks, 4>>(g, is_u8, 0, s);
+            if (qt == 2)
+                return launch_tile_topk<MfmaTile<ks, 2>>(g, is_u8, 0, s);
+            return launch_tile_topk<MfmaTile<ks, 1>>(g, is_u8, 0, s);
+        };
+        switch ((dims + 63) / 64) {
+        case 1: return go(std::integral_constant<int, 1>());
+        case 2: return go(std::integral_constant<int, 2>());
+        case 3: return go(std::integral_constant<int, 3>());
+        default: return go(std::integral_constant<int, 4>());
+        }
+    }
+    const uint32_t nch = nch_for_bytes(dims);
+    int rc = NBPG_OK;
+    int drc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
+        using T = DotTile<DT.value, NCH.value, 1>;
+        rc = launch_tile_topk<T>(g, 0, (size_t)T::RQ * (T::NW + 1) * 4, s);
+    });
+    return drc != NBPG_OK ? drc : rc;
+}
+
 template <typename Tile>
 static int launch_refrule(const RefGtArgs &g, uint32_t is_u8, size_t extra, hipStream_t s)
 {
@@ -363,6 +544,7 @@ int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *qu
     g.k = k;
     g.ids = ids;
     g.dists = dists;
+    g.row_stride = dims;
     const bool small = k <= 96; // 32 queries per workgroup still fit 64 KB of LDS
     if (use_mfma) {
         const uint32_t is_u8 = dtype == NBPG_U8;
