@@ -93,6 +93,13 @@ __global__ void mf_scatter_kernel(const uint32_t *__restrict__ cids, const float
         dists[o] = cd[t];
 }
 
+// 64-row blocks of the slice pass through LDS: the workgroup's 256 threads
+// load a block once (KS 16-byte pieces each, into registers while the
+// previous block is scored, then into the other LDS buffer) and every
+// wavefront reads its MFMA operands from there, so a base row is read from
+// memory once per 16 * MF_QW * MF_WAVES queries.
+#define MF_BLK 64
+
 template <int KS>
 __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
     const uint8_t *__restrict__ base, uint64_t n, const uint8_t *__restrict__ queries, uint64_t nq,
@@ -100,11 +107,13 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
     const uint64_t *__restrict__ thr, uint64_t slice_rows, uint32_t *__restrict__ cnt,
     uint64_t *__restrict__ cand, uint32_t mips)
 {
+    constexpr uint32_t PITCH = 64 * KS + 16; // row pitch in LDS (16-byte pad against bank conflicts)
+    __shared__ __align__(16) uint8_t rows_lds[2][MF_BLK * PITCH];
+    __shared__ int32_t xs_lds[2][MF_BLK];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint32_t row16 = lane % 16, kq = lane / 16;
     const uint64_t q0 = ((uint64_t)blockIdx.x * MF_WAVES + wv) * (16 * MF_QW);
-    if (q0 >= nq)
-        return;
+    const bool active = q0 < nq; // inactive waves still load and synchronise
     const uint32_t xmask = is_u8 ? 0x80808080u : 0u;
     // query fragments: lane supplies 16 bytes of query row (l % 16) at k-chunk (l / 16)
     v4i a[MF_QW][KS];
@@ -138,56 +147,83 @@ __global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
     }
     const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows;
     const uint64_t r1 = min(n, r0 + slice_rows);
-    // the next 16-row block's operands are loaded while this block is scored
-    v4i bn[KS];
-    uint32_t xnn = 0;
+    // this thread's pieces of a block: piece p = tid + 256 j is row p / (4 KS),
+    // 16-byte chunk p % (4 KS)
+    v4i stage[KS];
+    int32_t xstage = 0;
     auto fetch = [&](uint64_t b0) {
-        const uint64_t row = b0 + row16;
-        const bool rv = row < r1;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const uint32_t off = 64 * s + 16 * kq;
+        for (int j = 0; j < KS; ++j) {
+            const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
+            const uint64_t row = b0 + p / (4 * KS);
+            const uint32_t off = 16 * (p % (4 * KS));
             v4i v = {0, 0, 0, 0};
-            if (rv && off < dims) {
+            if (row < r1 && off < dims) {
                 const uint4 u = *reinterpret_cast<const uint4 *>(base + row * dims + off);
                 v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
             }
-            bn[s] = v;
+            stage[j] = v;
         }
-        xnn = rv ? nx[row] : 0u;
+        if (threadIdx.x < MF_BLK) {
+            const uint64_t row = b0 + threadIdx.x;
+            // a missing row fails every test
+            xstage = row < r1 ? (int32_t)nx[row] : (mips ? -(1 << 30) : (1 << 30));
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < KS; ++j) {
+            const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
+            *reinterpret_cast<v4i *>(&rows_lds[buf][(p / (4 * KS)) * PITCH + 16 * (p % (4 * KS))]) =
+                stage[j];
+        }
+        if (threadIdx.x < MF_BLK)
+            xs_lds[buf][threadIdx.x] = xstage;
     };
     fetch(r0);
-    for (uint64_t b0 = r0; b0 < r1; b0 += 16) {
-        const uint64_t row = b0 + row16;
-        const bool rv = row < r1;
-        v4i b[KS];
+    put(0);
+    __syncthreads();
+    int buf = 0;
+    for (uint64_t b0 = r0; b0 < r1; b0 += MF_BLK) {
+        const bool more = b0 + MF_BLK < r1;
+        if (more)
+            fetch(b0 + MF_BLK); // in flight while this block is scored
+        if (active) {
 #pragma unroll
-        for (int s = 0; s < KS; ++s)
-            b[s] = bn[s];
-        const uint32_t xn = xnn;
-        if (b0 + 16 < r1)
-            fetch(b0 + 16);
-        // a missing row fails every test
-        const int32_t xs = rv ? (int32_t)xn : (mips ? -(1 << 30) : (1 << 30));
+            for (int blk = 0; blk < MF_BLK / 16; ++blk) {
+                const uint32_t rl = 16 * blk + row16;
+                const uint64_t row = b0 + rl;
+                v4i b[KS];
 #pragma unroll
-        for (int t = 0; t < MF_QW; ++t) {
-            v4i c = {0, 0, 0, 0};
+                for (int s = 0; s < KS; ++s)
+                    b[s] = *reinterpret_cast<const v4i *>(&rows_lds[buf][rl * PITCH + 64 * s + 16 * kq]);
+                const int32_t xs = xs_lds[buf][rl];
 #pragma unroll
-            for (int s = 0; s < KS; ++s)
-                c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
+                for (int t = 0; t < MF_QW; ++t) {
+                    v4i c = {0, 0, 0, 0};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const bool adm = mips ? c[r] + xs >= U[t][r] : xs - 2 * c[r] <= U[t][r];
-                if (adm) {
-                    const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
-                                         : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
-                    const uint64_t qr = q0 + 16 * t + 4 * kq + r;
-                    const uint32_t p = atomicAdd(&cnt[qr], 1u);
-                    if (p < MF_CAND)
-                        cand[qr * MF_CAND + p] = make_key(d, (uint32_t)row);
+                    for (int s = 0; s < KS; ++s)
+                        c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const bool adm = mips ? c[r] + xs >= U[t][r] : xs - 2 * c[r] <= U[t][r];
+                        if (adm) {
+                            const uint32_t xn = (uint32_t)xs;
+                            const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
+                                                 : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
+                            const uint64_t qr = q0 + 16 * t + 4 * kq + r;
+                            const uint32_t p = atomicAdd(&cnt[qr], 1u);
+                            if (p < MF_CAND)
+                                cand[qr * MF_CAND + p] = make_key(d, (uint32_t)row);
+                        }
+                    }
                 }
             }
         }
+        if (more)
+            put(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -291,7 +327,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     const uint64_t tiles = (nq + qpb - 1) / qpb;
     uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / tiles);
     slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 4096));
-    const uint64_t slice_rows = ((n + slices - 1) / slices + 15) / 16 * 16;
+    const uint64_t slice_rows = ((n + slices - 1) / slices + MF_BLK - 1) / MF_BLK * MF_BLK;
     slices = (n + slice_rows - 1) / slice_rows;
     dim3 grid((uint32_t)tiles, (uint32_t)slices), block(64 * MF_WAVES);
     switch (KS) {
