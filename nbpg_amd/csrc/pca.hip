@@ -180,7 +180,8 @@ struct PcaArgs {
     float *y;
 };
 
-#define PCA_ROWS 32 // rows per wavefront (two 16-row MFMA tiles)
+#define PCA_TILES 4                // 16-row MFMA tiles per wavefront
+#define PCA_ROWS (16 * PCA_TILES) // rows per wavefront: W is read once per PCA_ROWS rows
 
 template <int XT, int NT>
 __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
@@ -191,17 +192,17 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
     if (row0 >= a.n)
         return;
     const uint32_t r16 = lane % 16, kg = lane / 16;
-    v4f acc[2][NT];
+    v4f acc[PCA_TILES][NT];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < PCA_TILES; ++t)
 #pragma unroll
         for (int j = 0; j < NT; ++j)
             acc[t][j] = v4f{0.f, 0.f, 0.f, 0.f};
     for (uint32_t k0 = 0; k0 < a.kp; k0 += 32) {
         const uint32_t kk = k0 + 8 * kg;
-        v8bf A[2];
+        v8bf A[PCA_TILES];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < PCA_TILES; ++t) {
             const uint64_t row = row0 + 16 * t + r16;
             float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             if (row < a.n && kk < a.d_in)
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
         for (int j = 0; j < NT; ++j) {
             const v8bf B = *reinterpret_cast<const v8bf *>(a.Wb + (uint64_t)(16 * j + r16) * a.kp + kk);
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
+            for (int t = 0; t < PCA_TILES; ++t)
                 acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t], B, acc[t][j], 0, 0, 0);
         }
     }
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
         const uint32_t col = 16 * j + r16;
         const float b = a.bias[col];
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < PCA_TILES; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint64_t row = row0 + 16 * t + 4 * kg + r;
