@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
 }
 
 // ---------------------------------------------------------------- FP64 stats
-#define ST_SLICES 64
+#define ST_SLICES 64 // fewest row slices of the column sums
 
 template <int XT>
 __global__ void pca_colsum_kernel(const void *__restrict__ x, uint64_t n, uint32_t d,
@@ -470,7 +470,12 @@ extern "C" int nbpg_pca_stats_f64(int x_dtype, const void *d_x, uint64_t n, uint
     if (!x_dtype_ok(x_dtype) || !d_x || !d_mean || n < 2 || d == 0 || d > 65535 * 64)
         return set_error(NBPG_EINVAL, "bad argument (n >= 2 rows needed)");
     hipStream_t s = as_stream(stream);
-    const uint64_t slice_rows = (n + ST_SLICES - 1) / ST_SLICES;
+    // row slices: enough workgroups to keep every CU streaming (each thread
+    // walks its column down one slice), at least 256 rows each
+    const uint64_t col_blocks = (d + 255) / 256;
+    uint64_t want = std::max<uint64_t>(ST_SLICES, (uint64_t)g_dev.cu * 16 / col_blocks);
+    want = std::min<uint64_t>(want, std::max<uint64_t>(1, n / 256));
+    const uint64_t slice_rows = (n + want - 1) / want;
     const uint32_t slices = (uint32_t)((n + slice_rows - 1) / slice_rows);
     double *part = nullptr;
     if (hipMallocAsync((void **)&part, (size_t)slices * d * 8, s) != hipSuccess)
