@@ -74,7 +74,10 @@ __device__ __forceinline__ void to_uniform(const uint4 (&qv)[NCH], uint32_t (&qs
     }
 }
 
-// <x, q> over NCH 16-byte chunks of the row; q wave-uniform
+// <x, q> over NCH 16-byte chunks of the row; q wave-uniform.  Two
+// accumulators take alternate words: a dot op that reads the previous dot's
+// result as its accumulator needs a wait state, two interleaved chains need
+// none (integer sums: the split is exact)
 template <int DT, int NCH>
 __device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row,
                                             const uint32_t (&q)[4 * NCH])
@@ -83,15 +86,15 @@ __device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row,
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
         x[c] = row[c];
-    uint32_t acc = 0;
+    uint32_t a0 = 0, a1 = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        acc = dot_traits<DT>::dot(x[c].x, q[4 * c + 0], acc);
-        acc = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], acc);
-        acc = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], acc);
-        acc = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], acc);
+        a0 = dot_traits<DT>::dot(x[c].x, q[4 * c + 0], a0);
+        a1 = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], a1);
+        a0 = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], a0);
+        a1 = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], a1);
     }
-    return acc;
+    return a0 + a1;
 }
 
 // exact reference distance as float: (float)(int32) sum of squares, or
@@ -100,10 +103,27 @@ __device__ __forceinline__ uint32_t dot_row(const uint4 *__restrict__ row,
 template <int DT>
 __device__ __forceinline__ float dist_from(uint32_t nx, uint32_t nq, uint32_t dot, uint32_t mips = 0)
 {
-    if (mips)
-        return -(float)(int32_t)(dot * dot_traits<DT>::scale);
-    uint32_t s = nx + nq - 2u * dot; // wraps back into range: true value < 2^31
-    return (float)(int32_t)(s * dot_traits<DT>::scale);
+    // both forms computed and one selected: a uniform branch here would split
+    // the callers' dot chains apart
+    const uint32_t s = nx + nq - 2u * dot; // wraps back into range: true value < 2^31
+    const float f = (float)(int32_t)((mips ? dot : s) * dot_traits<DT>::scale);
+    return mips ? -f : f;
+}
+
+// the four sums of one 16-byte chunk into two <x, q> and two |x|^2
+// accumulators (each chain's next use is two dot ops away)
+template <int DT>
+__device__ __forceinline__ void dot_chunk(const uint4 &x, uint32_t q0, uint32_t q1, uint32_t q2,
+                                          uint32_t q3, uint32_t (&xq)[2], uint32_t (&xx)[2])
+{
+    xq[0] = dot_traits<DT>::dot(x.x, q0, xq[0]);
+    xx[0] = dot_traits<DT>::dot(x.x, x.x, xx[0]);
+    xq[1] = dot_traits<DT>::dot(x.y, q1, xq[1]);
+    xx[1] = dot_traits<DT>::dot(x.y, x.y, xx[1]);
+    xq[0] = dot_traits<DT>::dot(x.z, q2, xq[0]);
+    xx[0] = dot_traits<DT>::dot(x.z, x.z, xx[0]);
+    xq[1] = dot_traits<DT>::dot(x.w, q3, xq[1]);
+    xx[1] = dot_traits<DT>::dot(x.w, x.w, xx[1]);
 }
 
 // distance to a row without a stored norm: |x|^2 comes from the same loaded
@@ -116,19 +136,11 @@ __device__ __forceinline__ float dist_row(const uint4 *__restrict__ row, const u
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
         x[c] = row[c];
-    uint32_t xq = 0, xx = 0;
+    uint32_t xq[2] = {0, 0}, xx[2] = {0, 0};
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        xq = dot_traits<DT>::dot(x[c].x, q[4 * c + 0], xq);
-        xq = dot_traits<DT>::dot(x[c].y, q[4 * c + 1], xq);
-        xq = dot_traits<DT>::dot(x[c].z, q[4 * c + 2], xq);
-        xq = dot_traits<DT>::dot(x[c].w, q[4 * c + 3], xq);
-        xx = dot_traits<DT>::dot(x[c].x, x[c].x, xx);
-        xx = dot_traits<DT>::dot(x[c].y, x[c].y, xx);
-        xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
-        xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
-    }
-    return dist_from<DT>(xx, nq, xq, mips);
+    for (int c = 0; c < NCH; ++c)
+        dot_chunk<DT>(x[c], q[4 * c], q[4 * c + 1], q[4 * c + 2], q[4 * c + 3], xq, xx);
+    return dist_from<DT>(xx[0] + xx[1], nq, xq[0] + xq[1], mips);
 }
 
 // same, with the query read from LDS one 16-byte chunk at a time (every lane
@@ -141,35 +153,28 @@ __device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, con
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
         x[c] = row[c];
-    uint32_t xq = 0, xx = 0;
+    uint32_t xq[2] = {0, 0}, xx[2] = {0, 0};
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const uint4 qc = q[c];
-        xq = dot_traits<DT>::dot(x[c].x, qc.x, xq);
-        xq = dot_traits<DT>::dot(x[c].y, qc.y, xq);
-        xq = dot_traits<DT>::dot(x[c].z, qc.z, xq);
-        xq = dot_traits<DT>::dot(x[c].w, qc.w, xq);
-        xx = dot_traits<DT>::dot(x[c].x, x[c].x, xx);
-        xx = dot_traits<DT>::dot(x[c].y, x[c].y, xx);
-        xx = dot_traits<DT>::dot(x[c].z, x[c].z, xx);
-        xx = dot_traits<DT>::dot(x[c].w, x[c].w, xx);
+        dot_chunk<DT>(x[c], qc.x, qc.y, qc.z, qc.w, xq, xx);
     }
-    return dist_from<DT>(xx, nq, xq, mips);
+    return dist_from<DT>(xx[0] + xx[1], nq, xq[0] + xq[1], mips);
 }
 
 template <int DT, int NCH>
 __device__ __forceinline__ uint32_t norm_regs(const uint4 (&q)[NCH])
 {
-    // |q|^2 of a row held in vector registers
-    uint32_t acc = 0;
+    // |q|^2 of a row held in vector registers (two interleaved chains)
+    uint32_t a0 = 0, a1 = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        acc = dot_traits<DT>::dot(q[c].x, q[c].x, acc);
-        acc = dot_traits<DT>::dot(q[c].y, q[c].y, acc);
-        acc = dot_traits<DT>::dot(q[c].z, q[c].z, acc);
-        acc = dot_traits<DT>::dot(q[c].w, q[c].w, acc);
+        a0 = dot_traits<DT>::dot(q[c].x, q[c].x, a0);
+        a1 = dot_traits<DT>::dot(q[c].y, q[c].y, a1);
+        a0 = dot_traits<DT>::dot(q[c].z, q[c].z, a0);
+        a1 = dot_traits<DT>::dot(q[c].w, q[c].w, a1);
     }
-    return acc;
+    return a0 + a1;
 }
 
 // ---------------------------------------------------------------- hashing

@@ -139,8 +139,9 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
     uint32_t *sel = reinterpret_cast<uint32_t *>(cs + 64 * IPL);
     const uint32_t R = a.R;
 
+    bool first = true;
     while (true) {
-        const uint32_t qi = next_query(a.counter, a.nq, lane);
+        const uint32_t qi = next_query(a.counter, a.nq, lane, first);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;

@@ -40,22 +40,31 @@ struct SearchArgs {
 
 #define FAST_FLAG 0x80000000ull // bit 31 of the id word: entry expanded
 
-// next query of a persistent launch (>= nq: none left).  counter[0] hands out
-// queries; counter[48] (another 128-byte line, so the exits do not contend
-// with the hand-out) counts the wavefronts that found none, and the last of
-// them returns both to zero, so consecutive launches on one counter (eager
-// or replayed from a graph) need no clearing step between them.  Counters
-// start at zero when allocated.
-__device__ __forceinline__ uint32_t next_query(uint32_t *counter, uint64_t nq, uint32_t lane)
+// next query of a persistent launch (>= nq: none left).  A wavefront's first
+// query is its own index in the grid, taken without an atomic: every fetch
+// of one counter is serialised in the memory system (about 11 ns apart),
+// which at launch would start the grid's waves one by one over tens of
+// microseconds.  Later queries come from counter[0], offset by the grid's
+// wave count.  counter[48] (another 128-byte line, so the exits do not
+// contend with the hand-out) counts the wavefronts that found none, and the
+// last of them returns both to zero, so consecutive launches on one counter
+// (eager or replayed from a graph) need no clearing step between them.
+// Counters start at zero when allocated.  `first` starts true per wave.
+__device__ __forceinline__ uint32_t next_query(uint32_t *counter, uint64_t nq, uint32_t lane, bool &first)
 {
+    const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
     uint32_t qi = 0;
+    const bool from_grid = first;
+    first = false;
+    if (from_grid)
+        qi = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
     if (lane == 0) {
-        qi = atomicAdd(counter, 1u);
-        // the exit count is taken after this atomic has returned (its value
-        // decides the branch), so no fence is needed -- and a device-scope
-        // fence here would write back and invalidate this XCD's L2
+        if (!from_grid)
+            qi = waves + atomicAdd(counter, 1u);
+        // the exit count is taken after the hand-out atomic has returned (its
+        // value decides the branch), so no fence is needed -- and a
+        // device-scope fence here would write back and invalidate this XCD's L2
         if (qi >= nq) {
-            const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
             if (atomicAdd(counter + 48, 1u) == waves - 1) {
                 atomicExch(counter, 0u);
                 atomicExch(counter + 48, 0u);

@@ -79,8 +79,9 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
     const uint32_t R = a.R;
     constexpr uint64_t EMPTY = ~0ull;
 
+    bool first = true;
     while (true) {
-        const uint32_t qi = next_query(a.counter, a.nq, lane);
+        const uint32_t qi = next_query(a.counter, a.nq, lane, first);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
@@ -296,8 +297,9 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
     const uint32_t c = lane % LPR, rsub = lane / LPR;
     constexpr uint64_t EMPTY = ~0ull;
 
+    bool first = true;
     while (true) {
-        const uint32_t qi = next_query(a.counter, a.nq, lane);
+        const uint32_t qi = next_query(a.counter, a.nq, lane, first);
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
