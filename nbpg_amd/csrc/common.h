@@ -154,10 +154,15 @@ __device__ __forceinline__ float dist_row_lds(const uint4 *__restrict__ row, con
     for (int c = 0; c < NCH; ++c)
         x[c] = row[c];
     uint32_t xq[2] = {0, 0}, xx[2] = {0, 0};
+    // the query chunks are read one step ahead of their use, not all at once:
+    // a whole query row in registers would cost 4 * NCH more of them
+    uint4 qc = q[0];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        const uint4 qc = q[c];
+        const uint4 qn = q[c + 1 < NCH ? c + 1 : c];
+        __builtin_amdgcn_sched_barrier(0);
         dot_chunk<DT>(x[c], qc.x, qc.y, qc.z, qc.w, xq, xx);
+        qc = qn;
     }
     return dist_from<DT>(xx[0] + xx[1], nq, xq[0] + xq[1], mips);
 }

@@ -462,8 +462,8 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     if (groups > need)
         groups = (uint32_t)std::max<uint64_t>(1, need);
     dim3 g(groups), b(64 * c.waves_per_group);
-    if (nseeds > 1 && (!reg || split))
-        return set_error(NBPG_EUNSUP, "seeded FAST search needs L <= 256 and row_split = 0");
+    if (nseeds > 1 && !reg)
+        return set_error(NBPG_EUNSUP, "seeded FAST search needs L <= 256");
     if (reg)
         return launch_search_reg(ix->dtype, ix->nch, ipl, E, split, g, b, (uint32_t)lds, stream, a);
     int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {

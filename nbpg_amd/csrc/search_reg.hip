@@ -1,9 +1,10 @@
 // search_reg.hip -- fast search with the beam held in vector registers.
 //
-// One wavefront per query, pulled from an atomic work counter (persistent
-// grid).  The beam of L <= 64*E entries lives in registers: lane i holds
-// entries i, 64+i, ... as (dist, id) keys in ascending order across lanes,
-// bit 31 of the id word marking an expanded entry.  Per step:
+// One wavefront per query (persistent grid: a wave's first query is its grid
+// index, later ones come from an atomic work counter).  The beam of
+// L <= 64*E entries lives in registers: lane i holds entries i, 64+i, ... as
+// (dist, id) keys in ascending order across lanes; a wave-uniform bit mask
+// marks the expanded entries.  Per step:
 //   select   the W best unexpanded entries (one ballot per register row)
 //   gather   their adjacency rows (one contiguous R*4-byte load each)
 //   filter   a lossy direct-mapped visited filter in LDS (as the reference's,
@@ -23,9 +24,9 @@ namespace nbpg {
 __device__ __forceinline__ uint64_t shfl_up1(uint64_t v)
 {
     constexpr int WAVE_SHR1 = 0x138;
-    const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, WAVE_SHR1, 0xf, 0xf, false);
-    const uint32_t hi =
-        __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), WAVE_SHR1, 0xf, 0xf, false);
+    // bound_ctrl: lane 0, which has no source lane, reads 0
+    const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, WAVE_SHR1, 0xf, 0xf, true);
+    const uint32_t hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), WAVE_SHR1, 0xf, 0xf, true);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -36,16 +37,42 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The beam's "expanded" marks live in a wave-uniform bit mask xm (bit i of
+// xm[e] for entry i + 64e), so keys compare without masking a flag and a
+// step's selection is scalar bit work.  Entries at positions >= L hold keys
+// pushed out of the beam: never smaller than the tail, so they can neither
+// count as smaller than a candidate that beats the tail nor be selected.
 template <int E>
-__device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t c, uint32_t L, uint32_t lane)
+__device__ __forceinline__ void xm_insert(uint64_t (&xm)[E], uint32_t pos)
+{
+    // bits at positions >= pos move up one; the new entry at pos is unexpanded
+    uint64_t carry = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t lo = WAVE * e;
+        const uint64_t w = xm[e];
+        uint64_t nw = w;
+        if (pos <= lo) {
+            nw = (w << 1) | carry;
+        } else if (pos < lo + WAVE) {
+            const uint64_t keep = (1ull << (pos - lo)) - 1;
+            nw = (w & keep) | ((w & ~keep) << 1);
+        }
+        carry = pos < lo + WAVE ? (w >> 63) : 0;
+        xm[e] = nw;
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t (&xm)[E], uint64_t c, uint32_t L,
+                                            uint32_t lane)
 {
     bool dup = false;
     uint32_t pos = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint64_t u = unflag(b[e]);
-        dup |= ballot(u == c) != 0;
-        pos += (uint32_t)__popcll(ballot(u < c));
+        dup |= ballot(b[e] == c) != 0;
+        pos += (uint32_t)__popcll(ballot(b[e] < c));
     }
     if (dup || pos >= L)
         return;
@@ -53,16 +80,16 @@ __device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t c, uint32
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         uint64_t up = shfl_up1(b[e]);
-        const uint64_t last = readlane64(b[e], WAVE - 1);
-        if (lane == 0)
-            up = carry;
-        carry = last;
+        if (E > 1) {
+            const uint64_t last = readlane64(b[e], WAVE - 1);
+            if (lane == 0)
+                up = carry;
+            carry = last;
+        }
         const uint32_t gp = lane + WAVE * e;
-        uint64_t nb = gp > pos ? up : (gp == pos ? c : b[e]);
-        if (gp >= L)
-            nb = ~0ull;
-        b[e] = nb;
+        b[e] = gp > pos ? up : (gp == pos ? c : b[e]);
     }
+    xm_insert<E>(xm, pos);
 }
 
 template <int DT, int NCH, int IPL, int E>
@@ -112,19 +139,22 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         }
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
-        uint64_t b[E];
+        uint64_t b[E], xm[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            b[e] = EMPTY;
-        uint32_t cmps = 1;
+            b[e] = EMPTY, xm[e] = 0;
         wave_sync(); // filter cleared before the start nodes are marked
-        if (a.nseeds > 1) {
-            // seeds: one distance per lane, then lane r takes the seed of rank r
-            uint32_t sid = NBPG_SENT;
-            for (uint32_t j = 0; j < a.nseeds; ++j)
-                sid = lane == j ? a.sp[j] : sid;
+        // start nodes (the seeds, or the single entry): one distance per
+        // lane, then lane r takes the start node of rank r.  One distance
+        // call for both cases keeps the query reads in one place.
+        const uint32_t ns = a.nseeds > 1 ? a.nseeds : 1u;
+        {
+            uint32_t sid = lane == 0 ? a.entry : NBPG_SENT;
+            if (a.nseeds > 1)
+                for (uint32_t j = 0; j < a.nseeds; ++j)
+                    sid = lane == j ? a.sp[j] : sid;
             uint64_t key = EMPTY;
-            if (lane < a.nseeds) {
+            if (lane < ns) {
                 key = make_key(dist_row_lds<DT, NCH>(
                                    reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)sid * a.stride),
                                    ql, nqn, a.mips),
@@ -132,25 +162,16 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
                 filt[fast_slot(sid, a.flog2)] = sid;
             }
             uint32_t rank = 0;
-            for (uint32_t j = 0; j < a.nseeds; ++j)
+            for (uint32_t j = 0; j < ns; ++j)
                 rank += readlane64(key, (int)j) < key ? 1u : 0u;
-            // lanes past nseeds all rank nseeds; lane `rank` learns who holds that rank
+            // lanes past ns all rank ns; lane `rank` learns who holds that rank
             const uint32_t src = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
             const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src * 4), (int)(uint32_t)key);
             const uint32_t hi =
                 (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src * 4), (int)(uint32_t)(key >> 32));
-            b[0] = (lane < a.nseeds && lane < L) ? (((uint64_t)hi << 32) | lo) : EMPTY;
-            cmps = a.nseeds;
-        } else {
-            const uint32_t en = a.entry;
-            const float d0 = dist_row_lds<DT, NCH>(
-                reinterpret_cast<const uint4 *>(a.vecs + (uint64_t)en * a.stride), ql, nqn, a.mips);
-            wave_sync();
-            if (lane == 0) {
-                b[0] = make_key(d0, en);
-                filt[fast_slot(en, a.flog2)] = en;
-            }
+            b[0] = (lane < ns && lane < L) ? (((uint64_t)hi << 32) | lo) : EMPTY;
         }
+        uint32_t cmps = ns;
         uint32_t nexp = 0;
         wave_sync();
         while (nexp < a.limit) {
@@ -159,7 +180,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t gp = lane + WAVE * e;
-                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
+                uint64_t m = ballot(gp < L && b[e] != EMPTY) & ~xm[e];
                 while (m && got < a.W) {
                     const int l = __builtin_ctzll(m);
                     const uint32_t id = __builtin_amdgcn_readlane((uint32_t)b[e], l);
@@ -167,8 +188,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
                     s1 = got == 1 ? id : s1;
                     s2 = got == 2 ? id : s2;
                     s3 = got == 3 ? id : s3;
-                    if ((int)lane == l)
-                        b[e] |= FAST_FLAG;
+                    xm[e] |= 1ull << l;
                     ++got;
                     m &= m - 1;
                 }
@@ -221,13 +241,13 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             for (int e = 0; e < E; ++e)
                 if ((uint32_t)e == wl / WAVE)
                     tail = readlane64(b[e], (int)(wl % WAVE));
-            const uint64_t worst = unflag(tail);
+            const uint64_t worst = tail;
 #pragma unroll
             for (int t = 0; t < IPL; ++t) {
                 uint64_t m = ballot(key[t] < worst);
                 while (m) {
                     const int l = __builtin_ctzll(m);
-                    beam_insert<E>(b, readlane64(key[t], l), L, lane);
+                    beam_insert<E>(b, xm, readlane64(key[t], l), L, lane);
                     m &= m - 1;
                 }
             }
@@ -239,7 +259,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             if (gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
-                const uint64_t v = unflag(b[e]);
+                const uint64_t v = b[e];
                 a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
                 if (a.out_dists)
                     a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);
@@ -333,10 +353,10 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
         const uint32_t nqn = __builtin_amdgcn_readfirstlane(row_sum<LPR>(nqp));
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
-        uint64_t b[E];
+        uint64_t b[E], xm[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            b[e] = EMPTY;
+            b[e] = EMPTY, xm[e] = 0;
 
         // scores cl[0:nf) and inserts the ones that beat the beam tail
         auto score_list = [&](uint32_t nf) {
@@ -358,9 +378,11 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                 for (int e = 0; e < E; ++e)
                     if ((uint32_t)e == wl / WAVE)
                         tail = readlane64(b[e], (int)(wl % WAVE));
-                const uint64_t worst = unflag(tail);
+                const uint64_t worst = tail;
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
+                    if (base + g * RPI >= nf)
+                        break; // no rows left in this group (uniform)
                     const uint32_t r = base + g * RPI + rsub;
                     uint32_t xx = 0, xq = 0;
                     xx = dot_traits<DT>::dot(x[g].x, x[g].x, xx);
@@ -379,28 +401,37 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                     uint64_t m = ballot(c == 0 && r < nf && key < worst);
                     while (m) {
                         const int l = __builtin_ctzll(m);
-                        beam_insert<E>(b, readlane64(key, l), L, lane);
+                        beam_insert<E>(b, xm, readlane64(key, l), L, lane);
                         m &= m - 1;
                     }
                 }
             }
         };
 
+        // start nodes (the seeds, or the single entry) go through the same
+        // scoring as the neighbours; the beam starts empty, so all are kept
         wave_sync();
-        if (lane == 0) {
-            cl[0] = a.entry;
-            filt[fast_slot(a.entry, a.flog2)] = a.entry;
+        const uint32_t ns = a.nseeds > 1 ? a.nseeds : 1u;
+        {
+            uint32_t sid = lane == 0 ? a.entry : NBPG_SENT;
+            if (a.nseeds > 1)
+                for (uint32_t j = 0; j < a.nseeds; ++j)
+                    sid = lane == j ? a.sp[j] : sid;
+            if (lane < ns) {
+                cl[lane] = sid;
+                filt[fast_slot(sid, a.flog2)] = sid;
+            }
         }
         wave_sync();
-        score_list(1);
-        uint32_t nexp = 0, cmps = 1;
+        score_list(ns);
+        uint32_t nexp = 0, cmps = ns;
         wave_sync();
         while (nexp < a.limit) {
             uint32_t got = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t gp = lane + WAVE * e;
-                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
+                uint64_t m = ballot(gp < L && b[e] != EMPTY) & ~xm[e];
                 while (m && got < a.W) {
                     const int l = __builtin_ctzll(m);
                     const uint32_t idn = __builtin_amdgcn_readlane((uint32_t)b[e], l);
@@ -408,8 +439,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                     s1 = got == 1 ? idn : s1;
                     s2 = got == 2 ? idn : s2;
                     s3 = got == 3 ? idn : s3;
-                    if ((int)lane == l)
-                        b[e] |= FAST_FLAG;
+                    xm[e] |= 1ull << l;
                     ++got;
                     m &= m - 1;
                 }
@@ -455,7 +485,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
             if (gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
-                const uint64_t v = unflag(b[e]);
+                const uint64_t v = b[e];
                 a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
                 if (a.out_dists)
                     a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);

@@ -405,7 +405,8 @@ def test_fast_and_reference_modes_agree_mostly(gpu, small):
 def test_fast_search_seeds(gpu, small, L, nseeds):
     """FAST search from several start nodes: exact sorted distances, the
     seeds' own distances counted, recall at least that of the single entry,
-    and the argument checks (repeats, more than 64, row-split kernel)."""
+    the argument checks (repeats, more than 64), and the same search on the
+    row-split kernel."""
     N = gpu
     base, q, rows = small
     ix = make_index(N, base, N.U8, 32, rows)
@@ -432,8 +433,18 @@ def test_fast_search_seeds(gpu, small, L, nseeds):
         N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=[1, 2, 2])
     with pytest.raises(N.NbpgError):
         N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=list(range(65)))
-    with pytest.raises(N.NbpgError):
-        N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=[1, 2, 3], fast=(2, 1, 11, 0, 1))
+    # the row-split kernel from the same seeds: exact sorted distances, the
+    # seeds counted, recall on par with the one-lane-per-row kernel
+    si, sdist, sst = N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, start_points=seeds,
+                              fast=(2, 1, 11, 0, 1))
+    si, sdist, sst = si.cpu().numpy().view(np.uint32), sdist.cpu().numpy(), sst.cpu().numpy()
+    assert (np.diff(sdist, axis=1) >= 0).all()
+    for j in range(0, len(q), 9):
+        ref = np.array([O.distance(O.U8, base[i], q[j]) for i in si[j]], np.float32)
+        assert (ref == sdist[j]).all()
+        assert len(set(si[j].tolist())) == 10
+    assert (sst[:, 1] >= len(seeds)).all()
+    assert O.recall(si, gt, base, q, O.U8, 10) >= r_seed - 0.01
     ix.close()
 
 
