@@ -499,6 +499,21 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
     }
 }
 
+// Launches a persistent kernel with at most as many workgroups as the device
+// holds at once (registers and LDS both counted): a wave's first query is its
+// grid index, so a workgroup that has to wait for a free slot would start
+// its queries late.
+template <typename K>
+static void launch_resident(K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t stream,
+                            const SearchArgs &a)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block.x, lds) == hipSuccess &&
+        per_cu > 0)
+        grid.x = std::min<uint32_t>(grid.x, (uint32_t)per_cu * g_dev.cu);
+    hipLaunchKernelGGL(kernel, grid, block, lds, stream, a);
+}
+
 int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
                       dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a)
 {
@@ -507,9 +522,9 @@ int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool sp
 #define NBPG_REG_LAUNCH(I, EE)                                                                     \
     do {                                                                                           \
         if (split)                                                                                 \
-            hipLaunchKernelGGL((search_split_kernel<D, C, I, EE>), grid, block, lds, stream, a);  \
+            launch_resident(search_split_kernel<D, C, I, EE>, grid, block, lds, stream, a);        \
         else                                                                                       \
-            hipLaunchKernelGGL((search_reg_kernel<D, C, I, EE>), grid, block, lds, stream, a);    \
+            launch_resident(search_reg_kernel<D, C, I, EE>, grid, block, lds, stream, a);          \
     } while (0)
         const uint32_t sel = (ipl <= 1 ? 0 : ipl <= 2 ? 1 : 2) * 3 + (E <= 1 ? 0 : E <= 2 ? 1 : 2);
         switch (sel) {
