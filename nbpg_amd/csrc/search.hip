@@ -453,9 +453,12 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                          (unsigned long long)lds);
     uint32_t groups = c.groups;
     if (groups == 0) {
-        // fill the LDS of every CU, at most 32 waves per CU
+        // fill the LDS of every CU, at most 24 waves per CU: the gathers are
+        // bound by the row-request rate, and beyond 24 waves more queries in
+        // flight only lengthen each one (launches are also capped at what the
+        // device holds at once)
         uint64_t per_cu = std::max<uint64_t>(1, (uint64_t)g_dev.lds_max / lds);
-        per_cu = std::min<uint64_t>(per_cu, std::max<uint32_t>(1, 32 / c.waves_per_group));
+        per_cu = std::min<uint64_t>(per_cu, std::max<uint32_t>(1, 24 / c.waves_per_group));
         groups = (uint32_t)(per_cu * g_dev.cu);
     }
     uint64_t need = (nq + c.waves_per_group - 1) / c.waves_per_group;

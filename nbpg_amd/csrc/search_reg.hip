@@ -3,8 +3,8 @@
 // One wavefront per query (persistent grid: a wave's first query is its grid
 // index, later ones come from an atomic work counter).  The beam of
 // L <= 64*E entries lives in registers: lane i holds entries i, 64+i, ... as
-// (dist, id) keys in ascending order across lanes; a wave-uniform bit mask
-// marks the expanded entries.  Per step:
+// (dist, id) keys in ascending order across lanes, bit 31 of the id word
+// marking an expanded entry.  Per step:
 //   select   the W best unexpanded entries (one ballot per register row)
 //   gather   their adjacency rows (one contiguous R*4-byte load each)
 //   filter   a lossy direct-mapped visited filter in LDS (as the reference's,
@@ -16,6 +16,10 @@
 // until no unexpanded entry is left in the beam.  No LDS is touched by the
 // beam, so a step has two dependent global round trips and no LDS chains.
 #include "search_args.h"
+
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace nbpg {
 
@@ -37,42 +41,21 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
     return ((uint64_t)hi << 32) | lo;
 }
 
-// The beam's "expanded" marks live in a wave-uniform bit mask xm (bit i of
-// xm[e] for entry i + 64e), so keys compare without masking a flag and a
-// step's selection is scalar bit work.  Entries at positions >= L hold keys
-// pushed out of the beam: never smaller than the tail, so they can neither
-// count as smaller than a candidate that beats the tail nor be selected.
+// Inserts key c into the beam (ascending across lanes, entry i of row e in
+// lane i of b[e]).  Bit 31 of an entry's id word marks it expanded; keys
+// compare without it.  Entries at positions >= L hold keys pushed out of the
+// beam: never smaller than the tail, so they can neither count as smaller
+// than a candidate that beats the tail nor be selected.
 template <int E>
-__device__ __forceinline__ void xm_insert(uint64_t (&xm)[E], uint32_t pos)
-{
-    // bits at positions >= pos move up one; the new entry at pos is unexpanded
-    uint64_t carry = 0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t lo = WAVE * e;
-        const uint64_t w = xm[e];
-        uint64_t nw = w;
-        if (pos <= lo) {
-            nw = (w << 1) | carry;
-        } else if (pos < lo + WAVE) {
-            const uint64_t keep = (1ull << (pos - lo)) - 1;
-            nw = (w & keep) | ((w & ~keep) << 1);
-        }
-        carry = pos < lo + WAVE ? (w >> 63) : 0;
-        xm[e] = nw;
-    }
-}
-
-template <int E>
-__device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t (&xm)[E], uint64_t c, uint32_t L,
-                                            uint32_t lane)
+__device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t c, uint32_t L, uint32_t lane)
 {
     bool dup = false;
     uint32_t pos = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        dup |= ballot(b[e] == c) != 0;
-        pos += (uint32_t)__popcll(ballot(b[e] < c));
+        const uint64_t u = unflag(b[e]);
+        dup |= ballot(u == c) != 0;
+        pos += (uint32_t)__popcll(ballot(u < c));
     }
     if (dup || pos >= L)
         return;
@@ -89,7 +72,6 @@ __device__ __forceinline__ void beam_insert(uint64_t (&b)[E], uint64_t (&xm)[E],
         const uint32_t gp = lane + WAVE * e;
         b[e] = gp > pos ? up : (gp == pos ? c : b[e]);
     }
-    xm_insert<E>(xm, pos);
 }
 
 template <int DT, int NCH, int IPL, int E>
@@ -139,10 +121,10 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         }
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
-        uint64_t b[E], xm[E];
+        uint64_t b[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            b[e] = EMPTY, xm[e] = 0;
+            b[e] = EMPTY;
         wave_sync(); // filter cleared before the start nodes are marked
         // start nodes (the seeds, or the single entry): one distance per
         // lane, then lane r takes the start node of rank r.  One distance
@@ -180,7 +162,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t gp = lane + WAVE * e;
-                uint64_t m = ballot(gp < L && b[e] != EMPTY) & ~xm[e];
+                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
                 while (m && got < a.W) {
                     const int l = __builtin_ctzll(m);
                     const uint32_t id = __builtin_amdgcn_readlane((uint32_t)b[e], l);
@@ -188,7 +170,8 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
                     s1 = got == 1 ? id : s1;
                     s2 = got == 2 ? id : s2;
                     s3 = got == 3 ? id : s3;
-                    xm[e] |= 1ull << l;
+                    if ((int)lane == l)
+                        b[e] |= FAST_FLAG;
                     ++got;
                     m &= m - 1;
                 }
@@ -241,13 +224,13 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             for (int e = 0; e < E; ++e)
                 if ((uint32_t)e == wl / WAVE)
                     tail = readlane64(b[e], (int)(wl % WAVE));
-            const uint64_t worst = tail;
+            const uint64_t worst = unflag(tail);
 #pragma unroll
             for (int t = 0; t < IPL; ++t) {
                 uint64_t m = ballot(key[t] < worst);
                 while (m) {
                     const int l = __builtin_ctzll(m);
-                    beam_insert<E>(b, xm, readlane64(key[t], l), L, lane);
+                    beam_insert<E>(b, readlane64(key[t], l), L, lane);
                     m &= m - 1;
                 }
             }
@@ -259,7 +242,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             if (gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
-                const uint64_t v = b[e];
+                const uint64_t v = unflag(b[e]);
                 a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
                 if (a.out_dists)
                     a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);
@@ -353,10 +336,10 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
         const uint32_t nqn = __builtin_amdgcn_readfirstlane(row_sum<LPR>(nqp));
         for (uint32_t i = lane; i < FS; i += WAVE)
             filt[i] = NBPG_SENT;
-        uint64_t b[E], xm[E];
+        uint64_t b[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            b[e] = EMPTY, xm[e] = 0;
+            b[e] = EMPTY;
 
         // scores cl[0:nf) and inserts the ones that beat the beam tail
         auto score_list = [&](uint32_t nf) {
@@ -378,7 +361,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                 for (int e = 0; e < E; ++e)
                     if ((uint32_t)e == wl / WAVE)
                         tail = readlane64(b[e], (int)(wl % WAVE));
-                const uint64_t worst = tail;
+                const uint64_t worst = unflag(tail);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     if (base + g * RPI >= nf)
@@ -401,7 +384,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                     uint64_t m = ballot(c == 0 && r < nf && key < worst);
                     while (m) {
                         const int l = __builtin_ctzll(m);
-                        beam_insert<E>(b, xm, readlane64(key, l), L, lane);
+                        beam_insert<E>(b, readlane64(key, l), L, lane);
                         m &= m - 1;
                     }
                 }
@@ -431,7 +414,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t gp = lane + WAVE * e;
-                uint64_t m = ballot(gp < L && b[e] != EMPTY) & ~xm[e];
+                uint64_t m = ballot(gp < L && b[e] != EMPTY && !(b[e] & FAST_FLAG));
                 while (m && got < a.W) {
                     const int l = __builtin_ctzll(m);
                     const uint32_t idn = __builtin_amdgcn_readlane((uint32_t)b[e], l);
@@ -439,7 +422,8 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
                     s1 = got == 1 ? idn : s1;
                     s2 = got == 2 ? idn : s2;
                     s3 = got == 3 ? idn : s3;
-                    xm[e] |= 1ull << l;
+                    if ((int)lane == l)
+                        b[e] |= FAST_FLAG;
                     ++got;
                     m &= m - 1;
                 }
@@ -485,7 +469,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
             if (gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
-                const uint64_t v = b[e];
+                const uint64_t v = unflag(b[e]);
                 a.out_ids[o] = ok ? key_id(v) : NBPG_SENT;
                 if (a.out_dists)
                     a.out_dists[o] = ok ? key_dist(v) : __int_as_float(0x7f800000);
@@ -503,13 +487,31 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
 // holds at once (registers and LDS both counted): a wave's first query is its
 // grid index, so a workgroup that has to wait for a free slot would start
 // its queries late.
+// The occupancy query is answered once per (device, kernel, block, LDS) and
+// kept: serving loops launch small batches at a high rate.
 template <typename K>
 static void launch_resident(K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t stream,
                             const SearchArgs &a)
 {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void *, uint32_t, uint32_t>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(dev, reinterpret_cast<const void *>(kernel), block.x, lds);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block.x, lds) == hipSuccess &&
-        per_cu > 0)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            per_cu = it->second;
+        } else {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block.x, lds) !=
+                hipSuccess)
+                per_cu = 0;
+            cache[key] = per_cu;
+        }
+    }
+    if (per_cu > 0)
         grid.x = std::min<uint32_t>(grid.x, (uint32_t)per_cu * g_dev.cu);
     hipLaunchKernelGGL(kernel, grid, block, lds, stream, a);
 }
