@@ -55,6 +55,19 @@ def test_rerank_matches_oracle(gpu, raw_kind):
         ei, ed = expected(raw, rq, cand.cpu().numpy().view(np.uint32), k, factor, rdt)
         assert (ids.cpu().numpy().view(np.uint32) == ei).all(), (k, factor)
         assert (d.cpu().numpy() == ed).all(), (k, factor)
+    # short frontiers (at most 32 candidates: several queries share a
+    # wavefront), some ended early, k above the candidate count
+    c16 = np.ascontiguousarray(cand.cpu().numpy().view(np.uint32)[:, :16])
+    c16[5, 9:] = 0xFFFFFFFF
+    c16[6, :] = 0xFFFFFFFF
+    c16[149, 3:] = 0xFFFFFFFF
+    cd16 = torch.from_numpy(c16.view(np.int32)).cuda()
+    for k, factor in [(10, 1), (10, 2), (4, 1), (16, 1), (20, 1), (1, 1), (3, 3)]:
+        ids, d = N.rerank(torch.from_numpy(raw).cuda(), torch.from_numpy(rq).cuda(), cd16, k,
+                          factor)
+        ei, ed = expected(raw, rq, c16, k, factor, rdt)
+        assert (ids.cpu().numpy().view(np.uint32) == ei).all(), (k, factor)
+        assert (d.cpu().numpy() == ed).all(), (k, factor)
     # the rerank can only improve the raw-space recall of the frontier head
     gt, _ = O.compgt(raw, rq, 10, rdt, tie_mode=1) if raw_kind == "u8" else (None, None)
     if gt is not None:
