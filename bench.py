@@ -234,6 +234,33 @@ def main():
     qps = nq * args.steps / wall
     log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
         (L_fast, qps, np.percentile(per, 50), np.percentile(per, 99)))
+    # end to end: queries from pinned host memory, search, ids and distances
+    # back to pinned host memory, one synchronous batch per step (host clock)
+    qh = torch.from_numpy(qpad).pin_memory()
+    q_e2e = torch.empty_like(q_d)
+    out_e2e = run_fast(L_fast)
+    ids_h = torch.empty(tuple(out_e2e[0].shape), dtype=out_e2e[0].dtype, pin_memory=True)
+    d_h = torch.empty(tuple(out_e2e[1].shape), dtype=out_e2e[1].dtype, pin_memory=True)
+
+    def e2e_step():
+        q_e2e.copy_(qh, non_blocking=True)
+        N.search(ix, q_e2e, L_fast, 10, mode=N.SEARCH_FAST, fast=fast, out=out_e2e,
+                 start_points=seeds)
+        ids_h.copy_(out_e2e[0], non_blocking=True)
+        d_h.copy_(out_e2e[1], non_blocking=True)
+        torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        e2e_step()
+    e2e = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        e2e_step()
+        e2e.append(time.perf_counter() - t0)
+    e2e = np.array(e2e) * 1e3
+    if not np.array_equal(ids_h.numpy(), out_e2e[0].cpu().numpy()):
+        raise SystemExit("end-to-end results differ from the device copy")
+    log("end to end (query upload, search, result download): %.0f QPS, p50 %.3f ms p99 %.3f ms"
+        % (nq / (e2e.mean() * 1e-3), np.percentile(e2e, 50), np.percentile(e2e, 99)))
     e0_qps = qps
     if cmp_e0:
         w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2, not args.no_graph)
@@ -283,6 +310,11 @@ def main():
         "cpu_baseline": cpu,
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
+        "end_to_end": {"qps": round(nq / (float(e2e.mean()) * 1e-3), 1),
+                       "p50_ms": round(float(np.percentile(e2e, 50)), 4),
+                       "p99_ms": round(float(np.percentile(e2e, 99)), 4),
+                       "what": "query rows from pinned host memory, search, ids and distances "
+                               "back to pinned host memory, synchronised per batch (host clock)"},
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
                   "speedup_fast_over_plain": round(qps / plain_qps, 2)},
         "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)}
