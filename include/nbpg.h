@@ -99,7 +99,9 @@ typedef struct {
     uint32_t waves_per_group; /* queries per workgroup: 1..16 */
     uint32_t expand_width;    /* beam entries expanded per step: 1..4 */
     uint32_t filter_log2;     /* LDS visited-filter slots = 1 << filter_log2 (8..14) */
-    uint32_t groups;          /* persistent workgroups; 0 = derive from occupancy */
+    uint32_t groups;          /* persistent workgroups; 0 = derive from occupancy (LDS, at
+                                 most 24 waves per CU); always capped at what the device
+                                 holds at once, as a wave's first query is its grid index */
     uint32_t row_split;       /* 0: one lane gathers a whole vector row; 1: one lane per
                                  16-byte chunk, a load instruction covers whole rows */
 } nbpg_fast_config;
