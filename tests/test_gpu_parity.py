@@ -468,3 +468,28 @@ def test_search_fast_large_beams(gpu, small, L, cfg):
         assert len(set(ids[j].tolist())) == 50
     assert O.recall(ids, gt, base, q, O.U8, 50) >= 0.97
     ix.close()
+
+
+@pytest.mark.parametrize("cfg", [(2, 1, 10, 0, 0), (4, 2, 11, 0, 0), (2, 1, 10, 0, 1)])
+def test_fast_results_independent_of_batch_position(gpu, small, cfg):
+    """A FAST query's result depends on the query, the index and the launch
+    shape only: permuting the batch, shrinking it or changing the number of
+    persistent workgroups (which changes which wave runs it, from its grid
+    index or the work counter) returns the same ids, distances and counters."""
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    seeds = N.pick_seeds(N.to_device(base), 16)
+    L = 40
+    ids, d, st = N.search(ix, dev_rows(N, q, ix.row_stride), L, 10, mode=N.SEARCH_FAST, fast=cfg,
+                          start_points=seeds)
+    ids, d, st = (t.cpu().numpy() for t in (ids, d, st))
+    perm = np.random.default_rng(3).permutation(len(q))
+    for qs, sel, groups in ((q[perm], perm, 0), (q[:37], np.arange(37), 0), (q, np.arange(len(q)), 3)):
+        c = (cfg[0], cfg[1], cfg[2], groups, cfg[4])
+        i2, d2, s2 = N.search(ix, dev_rows(N, qs, ix.row_stride), L, 10, mode=N.SEARCH_FAST,
+                              fast=c, start_points=seeds)
+        assert (i2.cpu().numpy() == ids[sel]).all()
+        assert (d2.cpu().numpy() == d[sel]).all()
+        assert (s2.cpu().numpy() == st[sel]).all()
+    ix.close()
