@@ -183,14 +183,13 @@ struct PcaArgs {
 #define PCA_TILES 4                // 16-row MFMA tiles per wavefront
 #define PCA_ROWS (16 * PCA_TILES) // rows per wavefront: W is read once per PCA_ROWS rows
 
-template <int XT, int NT>
-__global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
+// One wavefront's PCA_ROWS rows: acc = X_rows . W^T on bf16 MFMA, then bias,
+// y and codes.  W fragments come from Wp (row pitch wpitch elements), either
+// global memory or the workgroup's LDS copy.
+template <int XT, int NT, typename WPtr>
+__device__ __forceinline__ void pca_bf16_rows(const PcaArgs &a, uint64_t row0, WPtr Wp, uint32_t wpitch,
+                                              uint32_t lane)
 {
-    const uint32_t lane = threadIdx.x % WAVE;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint64_t row0 = wave * PCA_ROWS;
-    if (row0 >= a.n)
-        return;
     const uint32_t r16 = lane % 16, kg = lane / 16;
     v4f acc[PCA_TILES][NT];
 #pragma unroll
@@ -213,7 +212,7 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
         }
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            const v8bf B = *reinterpret_cast<const v8bf *>(a.Wb + (uint64_t)(16 * j + r16) * a.kp + kk);
+            const v8bf B = *reinterpret_cast<const v8bf *>(Wp + (uint64_t)(16 * j + r16) * wpitch + kk);
 #pragma unroll
             for (int t = 0; t < PCA_TILES; ++t)
                 acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t], B, acc[t][j], 0, 0, 0);
@@ -246,6 +245,43 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
                 }
             }
     }
+}
+
+// W read from global memory (through L1/L2) by every wavefront
+template <int XT, int NT>
+__global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
+{
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint64_t row0 = wave * PCA_ROWS;
+    if (row0 >= a.n)
+        return;
+    pca_bf16_rows<XT, NT>(a, row0, a.Wb, a.kp, lane);
+}
+
+// LDS variant: a workgroup of PCA_LDS_WAVES wavefronts copies W (16 NT rows
+// of kp bf16) into LDS, with a 16-byte pad per row against bank conflicts,
+// and each of its wavefronts then projects one row block reading W
+// fragments from LDS (one copy of W per PCA_LDS_WAVES blocks instead of one
+// read per block).
+#define PCA_LDS_WAVES 12
+template <int XT, int NT>
+__global__ __launch_bounds__(64 * PCA_LDS_WAVES) void pca_bf16_lds_kernel(PcaArgs a)
+{
+    extern __shared__ __align__(16) uint8_t lds_w[];
+    const uint32_t wpitch = a.kp + 8; // elements
+    const uint32_t row16s = a.kp / 8;  // 16-byte pieces per W row
+    for (uint32_t i = threadIdx.x; i < 16 * NT * row16s; i += blockDim.x) {
+        const uint32_t r = i / row16s, c = i % row16s;
+        *reinterpret_cast<uint4 *>(lds_w + ((uint64_t)r * wpitch + 8 * c) * 2) =
+            *reinterpret_cast<const uint4 *>(a.Wb + (uint64_t)r * a.kp + 8 * c);
+    }
+    __syncthreads();
+    const __bf16 *Wl = reinterpret_cast<const __bf16 *>(lds_w);
+    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const uint64_t row0 = ((uint64_t)blockIdx.x * (blockDim.x / WAVE) + wv) * PCA_ROWS;
+    if (row0 < a.n)
+        pca_bf16_rows<XT, NT>(a, row0, Wl, wpitch, lane);
 }
 
 // ---------------------------------------------------------------- FP64 stats
@@ -445,18 +481,31 @@ extern "C" int nbpg_pca_project(int x_dtype, const void *d_x, uint64_t n, uint32
     a.y = d_out_y;
     const uint64_t waves = (n + PCA_ROWS - 1) / PCA_ROWS;
     const dim3 grid((uint32_t)((waves + 3) / 4)), block(256);
+    // W in LDS when it fits (64 x 960 bf16: 124 KB) and the rows fill more
+    // than one workgroup per CU
+    const uint64_t lds_bytes = (uint64_t)d_out * (kp + 8) * 2;
+    const bool in_lds = lds_bytes <= (uint64_t)g_dev.lds_max && waves >= 2ull * g_dev.cu * PCA_LDS_WAVES;
+    const dim3 lgrid((uint32_t)((waves + PCA_LDS_WAVES - 1) / PCA_LDS_WAVES)), lblock(64 * PCA_LDS_WAVES);
     dispatch_x(x_dtype, [&](auto XT) {
         constexpr int X = XT.value;
+#define NBPG_PCA_LAUNCH(T)                                                                         \
+    do {                                                                                           \
+        if (in_lds)                                                                                \
+            hipLaunchKernelGGL((pca_bf16_lds_kernel<X, T>), lgrid, lblock, (uint32_t)lds_bytes, s, a); \
+        else                                                                                       \
+            hipLaunchKernelGGL((pca_bf16_kernel<X, T>), grid, block, 0, s, a);                     \
+    } while (0)
         switch (d_out / 16) {
-        case 1: hipLaunchKernelGGL((pca_bf16_kernel<X, 1>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((pca_bf16_kernel<X, 2>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((pca_bf16_kernel<X, 3>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((pca_bf16_kernel<X, 4>), grid, block, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((pca_bf16_kernel<X, 5>), grid, block, 0, s, a); break;
-        case 6: hipLaunchKernelGGL((pca_bf16_kernel<X, 6>), grid, block, 0, s, a); break;
-        case 7: hipLaunchKernelGGL((pca_bf16_kernel<X, 7>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((pca_bf16_kernel<X, 8>), grid, block, 0, s, a); break;
+        case 1: NBPG_PCA_LAUNCH(1); break;
+        case 2: NBPG_PCA_LAUNCH(2); break;
+        case 3: NBPG_PCA_LAUNCH(3); break;
+        case 4: NBPG_PCA_LAUNCH(4); break;
+        case 5: NBPG_PCA_LAUNCH(5); break;
+        case 6: NBPG_PCA_LAUNCH(6); break;
+        case 7: NBPG_PCA_LAUNCH(7); break;
+        default: NBPG_PCA_LAUNCH(8); break;
         }
+#undef NBPG_PCA_LAUNCH
     });
     const int rc = hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "bf16 projection launch");
     hipFreeAsync(Wb, s);

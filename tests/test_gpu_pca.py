@@ -107,6 +107,33 @@ def test_bf16_projection_error_bound(gpu, xt, d_in, d_out):
     print("bf16 %s %dx%d: max abs err %.3g, mean %.3g" % (xt, d_in, d_out, err.max(), err.mean()))
 
 
+@pytest.mark.parametrize("xt,d_in,d_out,code", [("f32", 96, 64, "CODE_I8"), ("u8", 64, 32, "CODE_U4"),
+                                                ("f32", 960, 64, "CODE_I4")])
+def test_bf16_projection_lds_path(gpu, xt, d_in, d_out, code):
+    """Large inputs take the persistent kernel with W in LDS; every 64-row
+    block is computed as by the global-W kernel (same operands, same MFMA
+    order), so its rows equal those of a small call over the same rows, bit
+    for bit, for y and the fused codes."""
+    N = gpu
+    import torch
+    rng = np.random.default_rng(d_in)
+    n = 420_000 if d_in < 960 else 410_000
+    if xt == "f32":
+        x = torch.randn((n, d_in), device="cuda", dtype=torch.float32) * 0.05
+    else:
+        x = torch.randint(0, 256, (n, d_in), device="cuda", dtype=torch.uint8)
+    W = torch.from_numpy((rng.standard_normal((d_out, d_in)) / (8.0 * np.sqrt(d_in) * (1 if xt == "f32" else 120))
+                          ).astype(np.float32)).cuda()
+    b = torch.from_numpy((rng.standard_normal(d_out) * 0.05).astype(np.float32)).cuda()
+    cd = getattr(N, code)
+    c, y = N.pca_project(x, W, b, code=cd, mode=N.PCA_BF16, want_y=True)
+    for lo in (0, 123_456, n - 1000):
+        cs, ys = N.pca_project(x[lo:lo + 1000].contiguous(), W, b, code=cd, mode=N.PCA_BF16,
+                               want_y=True)
+        # a row's MFMA sums do not depend on the other rows of its block
+        assert torch.equal(ys, y[lo:lo + 1000]) and torch.equal(cs, c[lo:lo + 1000]), lo
+
+
 def test_stats_f64(gpu):
     """FP64 mean and covariance against numpy's float64, d not a multiple of
     the 64-column tiles, f32 and u8 rows."""
