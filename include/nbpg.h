@@ -6,6 +6,9 @@
  * optional hipStream_t (passed as void*, NULL = the legacy default stream),
  * and returns an int status (NBPG_OK or a negative code; never aborts).
  * nbpg_last_error() gives the message of the calling thread's last failure.
+ * Calls on one index share its work counter and workspace: launches on one
+ * index must not overlap in time (issue them on one stream, or synchronise
+ * between streams); a server (nbpg_server_*) keeps counters of its own.
  *
  * Citations are file:line in the reference tree (godpgf/NBPG), naming the
  * interface each entry point stands in for.
