@@ -314,7 +314,14 @@ __global__ void pca_mean_kernel(const double *__restrict__ part, uint32_t slices
 }
 
 // one 64 x 64 tile (ti <= tj) of sum_r (x_r - mean)(x_r - mean)^T over a row
-// slice; wave w owns tile rows 16w..16w+15
+// slice; wave w owns tile rows 16w..16w+15.  Slabs of GR_K rows of both
+// column blocks are loaded once per workgroup (coalesced, the next slab in
+// registers while this one is summed), centred in FP64 and staged in LDS;
+// the MFMAs then run over the slab 4 rows at a time, in row order, so every
+// entry's sum is formed in the same order as a plain 4-rows-per-step loop.
+#define GR_K 32     // rows per slab
+#define GR_PITCH 66 // LDS row pitch in doubles (pad against bank conflicts)
+
 template <int XT>
 __global__ __launch_bounds__(256) void pca_gram_kernel(const void *__restrict__ x, uint64_t n,
                                                        uint32_t d, const double *__restrict__ mean,
@@ -322,36 +329,58 @@ __global__ __launch_bounds__(256) void pca_gram_kernel(const void *__restrict__ 
                                                        uint64_t slice_rows,
                                                        double *__restrict__ part)
 {
-    const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    __shared__ double sa[GR_K * GR_PITCH], sb[GR_K * GR_PITCH];
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const uint32_t r16 = lane % 16, kg = lane / 16;
     const uint32_t pr = pairs[blockIdx.x];
-    const uint32_t i0 = (pr >> 16) * 64 + 16 * wv, j0 = (pr & 0xFFFF) * 64;
-    const uint32_t ca = i0 + r16;
-    const bool va = ca < d;
-    const double ma = va ? mean[ca] : 0.0;
-    double mb[4];
-    bool vb[4];
+    const uint32_t ti = pr >> 16, tj = pr & 0xFFFF;
+    const bool diag = ti == tj;
+    const double *sbr = diag ? sa : sb;
+    // this thread's elements of a slab: rows 4j + tid / 64 (j < GR_K / 4), column tid % 64
+    const uint32_t col = tid % 64, roff = tid / 64;
+    const uint32_t ca = ti * 64 + col, cb = tj * 64 + col;
+    const double ma = ca < d ? mean[ca] : 0.0, mb = cb < d ? mean[cb] : 0.0;
+    const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows, r1 = min(n, r0 + slice_rows);
+    constexpr int J = GR_K / 4;
+    float fa[J], fb[J];
+    auto fetch = [&](uint64_t k0) {
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const uint32_t cb = j0 + 16 * jj + r16;
-        vb[jj] = cb < d;
-        mb[jj] = vb[jj] ? mean[cb] : 0.0;
-    }
+        for (int j = 0; j < J; ++j) {
+            const uint64_t row = k0 + 4 * j + roff;
+            fa[j] = (row < r1 && ca < d) ? x_at<XT>(x, row * d + ca) : 0.0f;
+            fb[j] = (!diag && row < r1 && cb < d) ? x_at<XT>(x, row * d + cb) : 0.0f;
+        }
+    };
+    auto put = [&](uint64_t k0) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint64_t row = k0 + 4 * j + roff;
+            const uint32_t at = (4 * j + roff) * GR_PITCH + col;
+            sa[at] = (row < r1 && ca < d) ? (double)fa[j] - ma : 0.0;
+            if (!diag)
+                sb[at] = (row < r1 && cb < d) ? (double)fb[j] - mb : 0.0;
+        }
+    };
     v4d acc[4];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj)
         acc[jj] = v4d{0.0, 0.0, 0.0, 0.0};
-    const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows, r1 = min(n, r0 + slice_rows);
-    for (uint64_t k0 = r0; k0 < r1; k0 += 4) {
-        const uint64_t row = k0 + kg;
-        const bool vr = row < r1;
-        const double av = (vr && va) ? (double)x_at<XT>(x, row * d + ca) - ma : 0.0;
+    fetch(r0);
+    for (uint64_t k0 = r0; k0 < r1; k0 += GR_K) {
+        put(k0);
+        __syncthreads();
+        if (k0 + GR_K < r1)
+            fetch(k0 + GR_K); // in flight while this slab is summed
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const double bv =
-                (vr && vb[jj]) ? (double)x_at<XT>(x, row * d + j0 + 16 * jj + r16) - mb[jj] : 0.0;
-            acc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[jj], 0, 0, 0);
+        for (int kk = 0; kk < GR_K; kk += 4) {
+            const double av = sa[(kk + kg) * GR_PITCH + 16 * wv + r16];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const double bv = sbr[(kk + kg) * GR_PITCH + 16 * jj + r16];
+                acc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[jj], 0, 0, 0);
+            }
         }
+        __syncthreads();
     }
     // the f64 16x16x4 result interleaves rows: register r of lane l holds
     // row 4r + l/16, column l%16 (unlike the 4(l/16) + r of the i8 / bf16
