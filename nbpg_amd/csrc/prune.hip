@@ -85,8 +85,19 @@ __global__ __launch_bounds__(64 * PRUNE_WAVES) void prune_kernel(PruneArgs a)
     const uint32_t m = wave_sort_candidates(keys, n, p, lane);
     const double alpha = a.alphas ? (double)a.alphas[t] : a.alpha;
     uint32_t comps = 0;
-    const uint32_t sel = wave_prune<DT, NCH>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride,
-                                             a.norms, oid, od, comps, lane, a.mips);
+    // up to 256 candidates of up to 128 bytes: their rows in registers
+    uint32_t sel;
+    if constexpr (NCH <= 8) {
+        if (m <= 4 * WAVE)
+            sel = wave_prune_regs<DT, NCH, 4>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride,
+                                              a.norms, oid, od, comps, lane, a.mips);
+        else
+            sel = wave_prune<DT, NCH>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride, a.norms,
+                                      oid, od, comps, lane, a.mips);
+    } else {
+        sel = wave_prune<DT, NCH>(keys, m, p, a.R, alpha, a.cos_angle, a.vecs, a.stride, a.norms, oid,
+                                  od, comps, lane, a.mips);
+    }
     if (lane == 0) {
         a.out_counts[t] = sel;
         if (a.out_comps)

@@ -142,4 +142,91 @@ __device__ uint32_t wave_prune(uint64_t *keys, uint32_t m, uint32_t p, uint32_t 
     return sel;
 }
 
+// The same scan with the candidates' rows held in registers: candidate
+// i = 64 k + lane lives in slot k of lane i % 64, loaded once, so the pass
+// of each selected p* reads only its own row from memory instead of every
+// later candidate's.  Needs m <= 64 * SLOTS.  Pairs, tests, distance counts
+// and pruned entries are exactly those of wave_prune.
+template <int DT, int NCH, int SLOTS>
+__device__ uint32_t wave_prune_regs(uint64_t *keys, uint32_t m, uint32_t p, uint32_t R, double alpha,
+                                    double cosa, const uint8_t *__restrict__ vecs, uint32_t stride,
+                                    const uint32_t *__restrict__ norms, uint32_t *out_ids, float *out_d,
+                                    uint32_t &comps, uint32_t lane, uint32_t mips = 0)
+{
+    const double eps = 1e-5;
+    uint4 rows[SLOTS][NCH];
+    uint32_t nrm[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+        const uint32_t i = WAVE * k + lane;
+        const uint32_t id = i < m ? key_id(keys[i]) : NBPG_SENT;
+        nrm[k] = 0;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            rows[k][c] = make_uint4(0, 0, 0, 0);
+        if (id != NBPG_SENT) {
+            const uint4 *r = reinterpret_cast<const uint4 *>(vecs + (uint64_t)id * stride);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                rows[k][c] = r[c];
+            nrm[k] = norms[id];
+        }
+    }
+    uint32_t sel = 0, scan = 0;
+    while (sel < R && scan < m) {
+        const uint64_t ks = keys[scan];
+        const uint32_t ps = key_id(ks);
+        ++scan;
+        if (ps == p || ps == NBPG_SENT)
+            continue;
+        const float d_ps = key_dist(ks);
+        if (lane == 0) {
+            out_ids[sel] = ps;
+            if (out_d)
+                out_d[sel] = d_ps;
+        }
+        ++sel;
+        uint4 sv[NCH];
+        const uint4 *sr = reinterpret_cast<const uint4 *>(vecs + (uint64_t)ps * stride);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            sv[c] = sr[c];
+        const uint32_t ns = norms[ps];
+        uint32_t ss[4 * NCH];
+        to_uniform<NCH>(sv, ss);
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            if (WAVE * (uint32_t)k + WAVE <= scan || WAVE * (uint32_t)k >= m)
+                continue; // uniform: the slot holds no candidate past p*
+            const uint32_t i = WAVE * k + lane;
+            const bool in = i >= scan && i < m;
+            const uint64_t kv = in ? keys[i] : ~0ull;
+            const bool live = in && key_id(kv) != NBPG_SENT;
+            bool prune = false;
+            if (live) {
+                uint32_t a0 = 0, a1 = 0;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    a0 = dot_traits<DT>::dot(rows[k][c].x, ss[4 * c + 0], a0);
+                    a1 = dot_traits<DT>::dot(rows[k][c].y, ss[4 * c + 1], a1);
+                    a0 = dot_traits<DT>::dot(rows[k][c].z, ss[4 * c + 2], a0);
+                    a1 = dot_traits<DT>::dot(rows[k][c].w, ss[4 * c + 3], a1);
+                }
+                const float d_sp = dist_from<DT>(nrm[k], ns, a0 + a1, mips);
+                const float d_pp = key_dist(kv);
+                // prune.h:227-238, alpha test in double
+                if (alpha > eps && alpha * (double)d_sp <= (double)d_pp)
+                    prune = true;
+                else if (cosa > eps && (double)cosine_at_p(d_ps, d_pp, d_sp) > cosa)
+                    prune = true;
+            }
+            comps += (uint32_t)__popcll(ballot(live));
+            if (prune)
+                keys[i] = (kv & 0xFFFFFFFF00000000ull) | NBPG_SENT;
+        }
+        wave_sync();
+    }
+    return sel;
+}
+
 } // namespace nbpg
