@@ -97,37 +97,44 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a)
     sort_and_write(a, q, kw, m, n2, lane);
 }
 
-// 16-byte aligned rows (row bytes % 16 == 0): 64 candidates at a time, their
-// rows staged through LDS 128 bytes per row per step with whole-line loads
-// (one load instruction covers 8 rows x 128 B), then each lane sums its own
-// row's piece from LDS in index order -- the same sequential sum as above.
-#define RS_CH 128                      // bytes of each row per step
-#define RS_PITCH 144                   // LDS row pitch (16-byte pad against bank conflicts)
-#define RS_STAGE (65 * RS_PITCH)       // 64 candidate rows + the query's piece
+// 16-byte aligned rows (row bytes % 16 == 0): up to 64 candidates at a time,
+// their rows staged through LDS a piece per row per step with whole-line
+// loads (one load instruction covers 8 rows x 128 B, or fewer rows with
+// longer pieces), then each lane sums its own row's piece from LDS in index
+// order -- the same sequential sum as above.  Row pitch in LDS: the piece
+// plus a 16-byte pad against bank conflicts.
+#define RS_STAGE (65 * 144) // 64 candidate rows of 128-byte pieces + the query's piece
 
 typedef uint32_t rs_vec __attribute__((ext_vector_type(4))); // a 16-byte piece
 
 // one step's loads: lane (sub, piece) takes piece `piece` (16 B) of rows
-// t * 8 + sub, t = 0..7, and lanes 0..7 the query's pieces
+// t * (64 / PPR) + sub, t = 0..7, and lanes 0..PPR-1 the query's pieces
+template <int PPR>
 __device__ __forceinline__ void rs_fetch(rs_vec (&buf)[8], rs_vec &qbuf, const uint8_t *base,
                                          uint64_t row_bytes, uint32_t my_id, const uint8_t *qrow,
                                          uint64_t c0, uint32_t gm, uint32_t piece, uint32_t sub)
 {
+    constexpr uint32_t RPI = WAVE / PPR; // rows per load instruction
     const uint64_t off = c0 + 16u * piece;
     const bool in_row = off < row_bytes;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        const uint32_t id = (uint32_t)__shfl((int)my_id, t * 8 + (int)sub);
-        if ((uint32_t)t * 8 + sub < gm && in_row)
+        const uint32_t id = (uint32_t)__shfl((int)my_id, t * RPI + (int)sub);
+        if ((uint32_t)t * RPI + sub < gm && in_row)
             buf[t] = *reinterpret_cast<const rs_vec *>(base + (uint64_t)id * row_bytes + off);
     }
     if (sub == 0 && in_row)
         qbuf = *reinterpret_cast<const rs_vec *>(qrow + off);
 }
 
-template <int RT, bool IP>
+// PPR 16-byte pieces of each row per step: G = 8 * 64 / PPR candidates per
+// pass (64 with PPR 8; few candidates take longer pieces, so a step still
+// moves 8 KB and a row takes fewer dependent steps)
+template <int RT, bool IP, int PPR>
 __global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32_t n2cap)
 {
+    constexpr uint32_t RPI = WAVE / PPR, G = 8 * RPI, CH = 16 * PPR, PITCH = CH + 16;
+    static_assert((G + 1) * PITCH <= RS_STAGE, "stage rows fit the per-wave stage");
     extern __shared__ __align__(16) uint8_t lds_rr[];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wv;
@@ -138,29 +145,29 @@ __global__ __launch_bounds__(256) void rerank_staged_kernel(RerankArgs a, uint32
     const uint8_t *qrow = a.queries + q * a.qstride;
     const uint32_t *cand = a.cand + q * a.cand_stride;
     const uint32_t m = frontier_len(a, q, lane), n2 = pow2_at_least(m);
-    const uint32_t piece = lane % 8, sub = lane / 8;
-    for (uint32_t g0 = 0; g0 < m; g0 += WAVE) {
-        const uint32_t gm = min((uint32_t)WAVE, m - g0);
+    const uint32_t piece = lane % PPR, sub = lane / PPR;
+    for (uint32_t g0 = 0; g0 < m; g0 += G) {
+        const uint32_t gm = min(G, m - g0);
         const uint32_t my_id = lane < gm ? cand[g0 + lane] : 0u;
         // the next step's pieces are loaded into registers while this
         // step's are summed (two steps in flight per wave)
         rs_vec buf[8], qbuf = {0, 0, 0, 0};
         typename raw_acc<RT, IP>::T acc = 0;
-        rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, 0, gm, piece, sub);
-        for (uint64_t c0 = 0; c0 < a.row_bytes; c0 += RS_CH) {
+        rs_fetch<PPR>(buf, qbuf, a.base, a.row_bytes, my_id, qrow, 0, gm, piece, sub);
+        for (uint64_t c0 = 0; c0 < a.row_bytes; c0 += CH) {
             const bool in_row = c0 + 16u * piece < a.row_bytes;
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if ((uint32_t)t * 8 + sub < gm && in_row)
-                    *reinterpret_cast<rs_vec *>(stage + (t * 8 + sub) * RS_PITCH + 16u * piece) = buf[t];
+                if ((uint32_t)t * RPI + sub < gm && in_row)
+                    *reinterpret_cast<rs_vec *>(stage + (t * RPI + sub) * PITCH + 16u * piece) = buf[t];
             if (sub == 0 && in_row)
-                *reinterpret_cast<rs_vec *>(stage + 64 * RS_PITCH + 16u * piece) = qbuf;
-            if (c0 + RS_CH < a.row_bytes)
-                rs_fetch(buf, qbuf, a.base, a.row_bytes, my_id, qrow, c0 + RS_CH, gm, piece, sub);
+                *reinterpret_cast<rs_vec *>(stage + G * PITCH + 16u * piece) = qbuf;
+            if (c0 + CH < a.row_bytes)
+                rs_fetch<PPR>(buf, qbuf, a.base, a.row_bytes, my_id, qrow, c0 + CH, gm, piece, sub);
             wave_sync();
             if (lane < gm)
-                acc = raw_acc<RT, IP>::add(acc, stage + lane * RS_PITCH, stage + 64 * RS_PITCH,
-                                       (uint32_t)min<uint64_t>(RS_CH, a.row_bytes - c0));
+                acc = raw_acc<RT, IP>::add(acc, stage + lane * PITCH, stage + G * PITCH,
+                                           (uint32_t)min<uint64_t>(CH, a.row_bytes - c0));
             wave_sync();
         }
         if (lane < gm)
@@ -222,11 +229,23 @@ int rerank_launch(int raw_dtype, uint32_t mips, const void *d_raw_base, uint64_t
         const uint32_t per_wave = RS_STAGE + 8 * n2, wpg = 4 * per_wave <= 65536 ? 4 : 2;
         const dim3 sg((uint32_t)((nq + wpg - 1) / wpg)), sb(WAVE * wpg);
         const uint32_t lds = wpg * per_wave;
+        // few candidates: longer pieces of each row per step
+        const int ppr = nc <= 16 ? 32 : (nc <= 32 ? 16 : 8);
         auto go = [&](auto RT) {
-            if (mips)
-                hipLaunchKernelGGL((rerank_staged_kernel<RT.value, true>), sg, sb, lds, s, a, n2);
+#define NBPG_RS_LAUNCH(P)                                                                          \
+    do {                                                                                           \
+        if (mips)                                                                                  \
+            hipLaunchKernelGGL((rerank_staged_kernel<RT.value, true, P>), sg, sb, lds, s, a, n2);  \
+        else                                                                                       \
+            hipLaunchKernelGGL((rerank_staged_kernel<RT.value, false, P>), sg, sb, lds, s, a, n2); \
+    } while (0)
+            if (ppr == 32)
+                NBPG_RS_LAUNCH(32);
+            else if (ppr == 16)
+                NBPG_RS_LAUNCH(16);
             else
-                hipLaunchKernelGGL((rerank_staged_kernel<RT.value, false>), sg, sb, lds, s, a, n2);
+                NBPG_RS_LAUNCH(8);
+#undef NBPG_RS_LAUNCH
         };
         rerank_dispatch(raw_dtype, go);
     } else {
