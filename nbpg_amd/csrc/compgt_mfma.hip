@@ -24,7 +24,7 @@ namespace nbpg {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 #define MF_CAND 4096 // candidate slots per query
-#define MF_QW 4      // 16-query A tiles per wave
+#define MF_QW 2      // 16-query A tiles per wave (2: 106 VGPRs, 4 waves per SIMD)
 #define MF_WAVES 4
 
 int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
