@@ -37,21 +37,33 @@ int compgt_tiles(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride
 // per row, with x' = x ^ 0x80 = x - 128 (u8) or x (i8), over dims bytes:
 // L2: |x'|^2.  Inner product: half of what turns <x',q'> back into <x,q>,
 // <x,q> = <x',q'> + A(x) + A(q) with A(x) = 128 sum(x') + 8192 dims (u8), 0 (i8)
+// rows of dims bytes (a multiple of 16, 16-byte aligned): 8 lanes per row,
+// each summing 16-byte pieces 128 bytes apart, then an xor reduction over
+// the 8 (integer sums: the split is exact)
 __global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, uint32_t dims,
                                 int is_u8, uint32_t mips, uint32_t *__restrict__ out)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint32_t *w = reinterpret_cast<const uint32_t *>(rows + i * dims);
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = t / 8;
+    const uint32_t piece = (uint32_t)(t % 8);
+    const uint32_t xm = is_u8 ? 0x80808080u : 0u;
     int acc = 0;
-    for (uint32_t t = 0; t < dims / 4; ++t) {
-        const uint32_t v = is_u8 ? (w[t] ^ 0x80808080u) : w[t];
-        acc = __builtin_amdgcn_sdot4((int)v, mips ? 0x01010101 : (int)v, acc, false);
+    if (i < n)
+        for (uint32_t off = 16 * piece; off < dims; off += 128) {
+            const uint4 u = *reinterpret_cast<const uint4 *>(rows + i * dims + off);
+            const uint32_t w[4] = {u.x ^ xm, u.y ^ xm, u.z ^ xm, u.w ^ xm};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc = __builtin_amdgcn_sdot4((int)w[j], mips ? 0x01010101 : (int)w[j], acc, false);
+        }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    acc += __shfl_xor(acc, 4);
+    if (i < n && piece == 0) {
+        if (mips)
+            acc = is_u8 ? 128 * acc + 8192 * (int)dims : 0;
+        out[i] = (uint32_t)acc;
     }
-    if (mips)
-        acc = is_u8 ? 128 * acc + 8192 * (int)dims : 0;
-    out[i] = (uint32_t)acc;
 }
 
 // threshold key: the sample's k-th distance with the largest id, so every
@@ -318,9 +330,9 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     }
     hipLaunchKernelGGL(mf_thresholds_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
                        td, nq, k, thr, cnt, ovf);
-    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, base, n,
+    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((n + 31) / 32)), dim3(256), 0, s, base, n,
                        dims, is_u8, mips, nx);
-    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((nq + 31) / 32)), dim3(256), 0, s,
                        queries, nq, dims, is_u8, mips, nqv);
     // 2. MFMA scan: query tiles x base slices
     const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
