@@ -277,7 +277,8 @@ template <int NCH> struct row_split_shape {
 template <int LPR> __device__ __forceinline__ uint32_t row_sum(uint32_t p)
 {
     p += swz_xor<1>(p);
-    p += swz_xor<2>(p);
+    if (LPR >= 4)
+        p += swz_xor<2>(p);
     if (LPR >= 8)
         p += swz_xor<4>(p);
     if (LPR >= 16)

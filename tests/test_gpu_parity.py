@@ -493,3 +493,33 @@ def test_fast_results_independent_of_batch_position(gpu, small, cfg):
         assert (d2.cpu().numpy() == d[sel]).all()
         assert (s2.cpu().numpy() == st[sel]).all()
     ix.close()
+
+
+@pytest.mark.parametrize("gen", ["i8", "u4", "i4"])
+@pytest.mark.parametrize("cfg", [(2, 1, 10, 0, 0), (2, 1, 10, 0, 1), (4, 2, 11, 0, 0)])
+def test_search_fast_other_types(gpu, gen, cfg):
+    """FAST search over i8 and packed 4-bit rows, both kernels, from seeds:
+    exact distances per returned id (the oracle's), sorted, distinct ids,
+    recall against the oracle's ground truth."""
+    N = gpu
+    dt = {"i8": N.I8, "u4": N.U4X2, "i4": N.I4X2}[gen]
+    if gen == "i8":
+        base = D.mixture_i8(4000, 64, 30, 30.0, seed=15).view(np.uint8)
+        q = D.mixture_i8(80, 64, 30, 30.0, seed=16).view(np.uint8)
+    else:
+        base = D.mixture_u8(4000, 32, 30, 25.0, seed=15)
+        q = D.mixture_u8(80, 32, 30, 25.0, seed=16)
+    rows, _ = O.build_graph(base, dt, 24, 48, alpha=1.2)
+    ix = make_index(N, base, dt, 24, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    seeds = N.pick_seeds(N.to_device(base), 32) if gen != "i8" else None
+    ids, d, _ = N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST, fast=cfg, start_points=seeds)
+    ids, d = ids.cpu().numpy().view(np.uint32), d.cpu().numpy()
+    assert (np.diff(d, axis=1) >= 0).all()
+    for j in range(len(q)):
+        assert len(set(ids[j].tolist())) == 10
+        ref = np.array([O.distance(dt, base[i], q[j]) for i in ids[j]], np.float32)
+        assert (ref == d[j]).all(), j
+    gt, _ = O.compgt(base, q, 100, dt, tie_mode=1)
+    assert O.recall(ids, gt, base, q, dt, 10) >= 0.9, (gen, cfg)
+    ix.close()
