@@ -70,15 +70,18 @@ def test_mips_search_build_prune_match_reference(gpu, golden_mips, name, nt, ot)
         ids, d, st = N.search(ix, qd, int(L), 10, mode=N.SEARCH_REF)
         assert (ids.cpu().numpy().view(np.uint32) == z[name + "_search_ids"][i]).all(), int(L)
         assert (st.cpu().numpy().view(np.uint32) == z[name + "_search_stats"][i]).all(), int(L)
-        # FAST: valid ids, each with its own inner-product distance
-        fi, fd, _ = N.search(ix, qd, int(L), 10, mode=N.SEARCH_FAST)
-        fi, fd = fi.cpu().numpy().view(np.uint32), fd.cpu().numpy()
-        for r in range(0, len(q), 9):
-            for j in range(10):
-                if fi[r, j] != 0xFFFFFFFF:
-                    assert fd[r, j] == O.distance(ot | O.MIPS, base[fi[r, j]], q[r])
-            live_d = fd[r][fi[r] != 0xFFFFFFFF]
-            assert (np.diff(live_d) >= 0).all()
+        # FAST (one lane per row, then 16-byte pieces per lane): valid ids,
+        # each with its own inner-product distance
+        for cfg in (None, (2, 1, 10, 0, 1)):
+            fi, fd, _ = N.search(ix, qd, int(L), 10, mode=N.SEARCH_FAST, fast=cfg)
+            fi, fd = fi.cpu().numpy().view(np.uint32), fd.cpu().numpy()
+            for r in range(0, len(q), 9):
+                for j in range(10):
+                    if fi[r, j] != 0xFFFFFFFF:
+                        assert fd[r, j] == O.distance(ot | O.MIPS, base[fi[r, j]], q[r]), cfg
+                live_d = fd[r][fi[r] != 0xFFFFFFFF]
+                assert (np.diff(live_d) >= 0).all()
+                assert len(set(fi[r][fi[r] != 0xFFFFFFFF].tolist())) == len(live_d)
     ix.close()
     ix = mips_index(N, base, dt, 24)
     N.build(ix, 24, 48, alpha=1.2, passes=1, num_split=0)
