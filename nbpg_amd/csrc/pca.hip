@@ -301,16 +301,24 @@ __global__ void pca_colsum_kernel(const void *__restrict__ x, uint64_t n, uint32
     part[(uint64_t)blockIdx.y * d + col] = s;
 }
 
-__global__ void pca_mean_kernel(const double *__restrict__ part, uint32_t slices, uint32_t d,
-                                uint64_t n, double *__restrict__ mean)
+// column means from the slices' sums: 64 columns per workgroup, 4 threads
+// per column each summing every 4th slice, then the 4 partials in a fixed
+// order (deterministic)
+__global__ __launch_bounds__(256) void pca_mean_kernel(const double *__restrict__ part,
+                                                       uint32_t slices, uint32_t d, uint64_t n,
+                                                       double *__restrict__ mean)
 {
-    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= d)
-        return;
+    __shared__ double ps[4][64];
+    const uint32_t c = threadIdx.x % 64, g = threadIdx.x / 64;
+    const uint32_t col = blockIdx.x * 64 + c;
     double s = 0.0;
-    for (uint32_t i = 0; i < slices; ++i)
-        s += part[(uint64_t)i * d + col];
-    mean[col] = s / (double)n;
+    if (col < d)
+        for (uint32_t i = g; i < slices; i += 4)
+            s += part[(uint64_t)i * d + col];
+    ps[g][c] = s;
+    __syncthreads();
+    if (g == 0 && col < d)
+        mean[col] = (((ps[0][c] + ps[1][c]) + ps[2][c]) + ps[3][c]) / (double)n;
 }
 
 // one 64 x 64 tile (ti <= tj) of sum_r (x_r - mean)(x_r - mean)^T over a row
@@ -562,7 +570,7 @@ extern "C" int nbpg_pca_stats_f64(int x_dtype, const void *d_x, uint64_t n, uint
         hipLaunchKernelGGL((pca_colsum_kernel<XT.value>), dim3((d + 255) / 256, slices), dim3(256),
                            0, s, d_x, n, d, slice_rows, part);
     });
-    hipLaunchKernelGGL(pca_mean_kernel, dim3((d + 255) / 256), dim3(256), 0, s, part, slices, d, n,
+    hipLaunchKernelGGL(pca_mean_kernel, dim3((d + 63) / 64), dim3(256), 0, s, part, slices, d, n,
                        d_mean);
     hipFreeAsync(part, s);
     if (hipGetLastError() != hipSuccess)
