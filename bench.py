@@ -56,8 +56,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device (HBM counter calibration)")
-    ap.add_argument("--calibrate-gather", action="store_true",
-                    help="only measure random 128-byte row gathers (index_select)")
     ap.add_argument("--workload", default="search",
                     choices=["search", "c1", "pca", "uneven", "serve"],
                     help="search: the headline 1M x 128 u8 batched search (default); c1: 100k x "
@@ -84,9 +82,6 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "3 copies of 1 GiB", "bytes_read_per_copy": 1 << 30,
                           "bytes_written_per_copy": 1 << 30}), flush=True)
-        return
-    if args.calibrate_gather:
-        print(json.dumps(gather_ceiling(torch)), flush=True)
         return
     if args.workload != "search":
         import workloads  # noqa: E402  (bench-only module beside this file)
@@ -331,39 +326,6 @@ def main():
                      for r in (tune_table or [])],
     }
     print(json.dumps(res), flush=True)
-
-
-def gather_ceiling(torch, table_rows=(1 << 20, 3 << 20)):
-    """Random 128-B row gather rate of this device (calibration probe)."""
-    import ctypes as C
-    lib = C.CDLL(os.path.join(ROOT, "nbpg_amd", "libnbpg_probe.so"))
-    lib.nbpg_probe_gather.restype = C.c_int
-    lib.nbpg_probe_gather.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32,
-                                      C.c_uint32, C.c_void_p, C.c_void_p]
-    res = []
-    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    for rows in table_rows:
-        tab = torch.randint(0, 255, (rows, 128), dtype=torch.uint8, device="cuda")
-        for split in (0, 1):
-            for blocks in (2048, 4096, 8192):
-                iters = 64
-                out = torch.empty(blocks * 256, dtype=torch.int32, device="cuda")
-                run = lambda: lib.nbpg_probe_gather(tab.data_ptr(), rows, 128, split, blocks,  # noqa
-                                                    iters, out.data_ptr(), stream)
-                run()
-                torch.cuda.synchronize()
-                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                a.record()
-                for _ in range(10):
-                    run()
-                b.record()
-                torch.cuda.synchronize()
-                ms = a.elapsed_time(b) / 10
-                nrow = blocks * 256 * iters // (8 if split else 1)
-                res.append({"table_MB": rows * 128 >> 20, "split": split, "blocks": blocks,
-                            "GBs": round(nrow * 128 / ms / 1e6, 1)})
-    return {"calibration": "random 128-B row gathers", "runs": res,
-            "best_GBs": max(r["GBs"] for r in res)}
 
 
 GTD = None

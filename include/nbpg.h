@@ -194,7 +194,9 @@ int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq
                        const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
                        uint32_t *d_out_stats, void *stream);
 /* FAST mode with per-query beam width and k (uneven batches); d_L, d_k are
- * nq u32 each on the device, outputs have stride out_k per query */
+ * nq u32 each on the device, outputs have stride out_k per query.  A query's
+ * L is capped at max_L (and raised to 1), its k at min(L, out_k), by every
+ * kernel shape alike. */
 int nbpg_search_uneven(const nbpg_index *idx, const void *d_queries, uint64_t nq,
                        uint64_t query_stride, const uint32_t *d_L, const uint32_t *d_k,
                        uint32_t max_L, uint32_t out_k, uint32_t entry,
@@ -289,11 +291,16 @@ int nbpg_server_create(nbpg_server **out, const nbpg_index *idx, const nbpg_serv
 /* copies the rows; returns a ticket to wait on (once) */
 int nbpg_server_submit(nbpg_server *srv, const void *host_queries, uint32_t nq,
                        uint64_t query_stride, uint32_t L, uint32_t k, uint64_t *ticket);
-/* blocks until the request is done; nq x k ids / dists (either may be NULL) */
+/* blocks until the request is done; nq x k ids / dists (either may be NULL).
+ * One wait per ticket: a second wait on a ticket that a thread is already
+ * waiting on returns NBPG_EINVAL. */
 int nbpg_server_wait(nbpg_server *srv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
                      double *latency_ms);
 int nbpg_server_stats(nbpg_server *srv, int reset, nbpg_server_stats_t *stats);
-/* finishes queued requests, then frees the slots */
+/* finishes queued requests, lets threads already blocked in
+ * nbpg_server_wait take their results, then frees the slots and the results
+ * nobody waited for.  No submit or wait may START once destroy has been
+ * called. */
 int nbpg_server_destroy(nbpg_server *srv);
 
 /* ---- graph build: build_vamana (nsw.h:547-663) on the device; the

@@ -342,7 +342,10 @@ int nbpg_index_set_points(nbpg_index *ix, const void *host_rows, uint64_t host_s
                          (unsigned long long)ix->n, (unsigned long long)host_stride);
     if (count == 0)
         return NBPG_OK;
-    if (host_stride == ix->stride) {
+    // a flat copy only when a host row has no bytes past `dims`: padding
+    // bytes (the reference leaves them uninitialised, point_range.h:51-56)
+    // must not reach the device rows, whose tails every kernel sums as zeros
+    if (host_stride == ix->stride && ix->dims == ix->stride) {
         HIP_TRY(hipMemcpy(ix->vecs + first * ix->stride, host_rows, count * ix->stride,
                           hipMemcpyHostToDevice));
     } else {

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
-        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, (uint32_t)(WAVE * E))));
+        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, min(a.Lmax, (uint32_t)(WAVE * E)))));
         uint32_t k = a.ks ? a.ks[qi] : a.k;
         k = __builtin_amdgcn_readfirstlane(min(k, min(L, a.out_k)));
         // the query row into this wave's LDS: one word per lane with a
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
         if (qi >= a.nq)
             break;
         uint32_t L = a.Ls ? a.Ls[qi] : a.L;
-        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, (uint32_t)(WAVE * E))));
+        L = __builtin_amdgcn_readfirstlane(max(1u, min(L, min(a.Lmax, (uint32_t)(WAVE * E)))));
         uint32_t k = a.ks ? a.ks[qi] : a.k;
         k = __builtin_amdgcn_readfirstlane(min(k, min(L, a.out_k)));
         // this lane's 16-byte chunk of the query (zero past the row)

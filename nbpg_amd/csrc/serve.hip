@@ -42,6 +42,7 @@ struct Request {
     std::vector<float> dists;
     Clock::time_point t_submit, t_done;
     bool done = false;
+    bool waited = false; // a thread is inside nbpg_server_wait for it
     int status = NBPG_OK;
     std::string error;
 };
@@ -74,6 +75,7 @@ struct nbpg_server {
     std::deque<nbpg::Request *> pending;
     std::unordered_map<uint64_t, nbpg::Request *> live;
     uint64_t next_ticket = 1;
+    uint32_t waiters = 0; // threads inside nbpg_server_wait
     bool stop = false;
     std::thread worker;
     // statistics since creation or the last reset
@@ -321,8 +323,17 @@ int nbpg_server_wait(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float
     if (it == sv->live.end())
         return set_error(NBPG_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
     Request *r = it->second;
+    if (r->waited)
+        return set_error(NBPG_EINVAL, "ticket %llu is already being waited on",
+                         (unsigned long long)ticket);
+    r->waited = true;
+    sv->waiters++;
     sv->cv_done.wait(lk, [r] { return r->done; });
-    sv->live.erase(it);
+    // by key: submits while this thread slept may have rehashed the map
+    sv->live.erase(ticket);
+    sv->waiters--;
+    if (sv->stop)
+        sv->cv_done.notify_all(); // destroy waits for the last waiter to leave
     lk.unlock();
     int rc = r->status;
     if (rc == NBPG_OK) {
@@ -378,8 +389,17 @@ int nbpg_server_destroy(nbpg_server *sv)
     sv->cv_work.notify_all();
     if (sv->worker.joinable())
         sv->worker.join();
-    for (auto &kv : sv->live)
-        delete kv.second;
+    {
+        // every request is done now; threads already blocked in
+        // nbpg_server_wait take their results and leave before the
+        // requests nobody waits for are freed
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_done.notify_all();
+        sv->cv_done.wait(lk, [sv] { return sv->waiters == 0; });
+        for (auto &kv : sv->live)
+            delete kv.second;
+        sv->live.clear();
+    }
     for (Slot &s : sv->slots)
         free_slot(s);
     delete sv;

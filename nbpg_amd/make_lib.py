@@ -42,21 +42,6 @@ def build(jobs=None):
     return LIB
 
 
-PROBE_SRC = os.path.join(CSRC, "probe", "gather_probe.hip")
-PROBE_LIB = os.path.join(HERE, "libnbpg_probe.so")
-
-
-def build_probe():
-    """The roofline calibration probe (a separate library, not the C-ABI)."""
-    tmp = PROBE_LIB + ".tmp"
-    r = subprocess.run([HIPCC] + FLAGS + ["-shared", PROBE_SRC, "-o", tmp], capture_output=True,
-                       text=True)
-    if r.returncode != 0:
-        raise RuntimeError("probe build failed:\n" + r.stderr[-4000:])
-    os.replace(tmp, PROBE_LIB)
-    return PROBE_LIB
-
-
 TOOLS = {"Vamana": "vamana.cpp", "CompGT": "compgt.cpp", "PCACompress": "pcacompress.cpp"}
 BIN = os.path.join(HERE, "bin")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -85,6 +70,5 @@ def build_tools():
 
 if __name__ == "__main__":
     print(build())
-    print(build_probe())
     print(build_tools())
     sys.exit(0)

@@ -55,7 +55,7 @@ def test_set_points_strided_and_partial(gpu):
     (the staging + spread path) and give exact distances."""
     N = gpu
     rng = np.random.default_rng(4)
-    for dims, hstride in [(20, 24), (100, 100), (96, 112)]:
+    for dims, hstride in [(20, 24), (100, 100), (96, 112), (20, 32), (50, 64), (120, 128)]:
         wide = rng.integers(0, 256, (3000, hstride), dtype=np.uint8)
         base = wide[:, :dims]
         ix = N.Index(N.U8, 3000, dims, 8)
@@ -68,6 +68,39 @@ def test_set_points_strided_and_partial(gpu):
         out = N.l2_batch(ix, N.to_device(q), N.to_device(ids)).cpu().numpy()
         ref = np.array([O.distance(O.U8, base[i], q) for i in ids], np.float32)
         assert (out == ref).all(), dims
+        ix.close()
+
+
+def test_set_points_padded_rows_ignore_padding(gpu):
+    """Rows in the reference's padded layout (stride 64 * ceil(bytes / 64),
+    point_range.h:51-56, padding left uninitialised there) whose stride
+    equals the device row stride: random padding bytes must not reach the
+    distances, so l2_batch and a FAST search equal the oracle's."""
+    N = gpu
+    rng = np.random.default_rng(8)
+    for dims, hstride in [(20, 32), (50, 64), (120, 128)]:
+        base = D.mixture_u8(3000, dims, 20, 25.0, seed=dims)
+        q = D.mixture_u8(40, dims, 20, 25.0, seed=dims + 1)
+        wide = rng.integers(0, 256, (3000, hstride), dtype=np.uint8)
+        wide[:, :dims] = base
+        ix = N.Index(N.U8, 3000, dims, 24)
+        N._check(N.lib().nbpg_index_set_points(ix.h, wide.ctypes.data, hstride, 0, 3000),
+                 "set_points")
+        ids = np.arange(3000, dtype=np.uint32)
+        out = N.l2_batch(ix, N.to_device(q[0]), N.to_device(ids)).cpu().numpy()
+        ref = np.array([O.distance(O.U8, base[i], q[0]) for i in ids], np.float32)
+        assert (out == ref).all(), dims
+        rows, _ = O.build_graph(base, O.U8, 24, 48, alpha=1.2)
+        ix.set_graph(rows)
+        qd = dev_rows(N, q, ix.row_stride)
+        fi, fd, _ = N.search(ix, qd, 32, 10, mode=N.SEARCH_FAST)
+        fi, fd = fi.cpu().numpy().view(np.uint32), fd.cpu().numpy()
+        for j in range(len(q)):
+            r = np.array([O.distance(O.U8, base[i], q[j]) for i in fi[j]], np.float32)
+            assert (r == fd[j]).all(), (dims, j)
+        ri, _, _ = N.search(ix, qd, 32, 10, mode=N.SEARCH_REF)
+        oi, _, _ = O.search(rows, base, q, O.U8, 32, 10)
+        assert (ri.cpu().numpy().view(np.uint32) == oi).all(), dims
         ix.close()
 
 
@@ -512,7 +545,7 @@ def test_search_fast_other_types(gpu, gen, cfg):
     rows, _ = O.build_graph(base, dt, 24, 48, alpha=1.2)
     ix = make_index(N, base, dt, 24, rows)
     qd = dev_rows(N, q, ix.row_stride)
-    seeds = N.pick_seeds(N.to_device(base), 32) if gen != "i8" else None
+    seeds = N.pick_seeds(N.to_device(base), 32, dtype=dt) if gen != "i8" else None
     ids, d, _ = N.search(ix, qd, 48, 10, mode=N.SEARCH_FAST, fast=cfg, start_points=seeds)
     ids, d = ids.cpu().numpy().view(np.uint32), d.cpu().numpy()
     assert (np.diff(d, axis=1) >= 0).all()

@@ -20,6 +20,7 @@ def served(gpu):
     ix = N.Index(N.U8, base.shape[0], base.shape[1], 32)
     ix.set_points(base)
     N.build(ix, 32, 64, alpha=1.2, passes=1, num_split=0)
+    ix.host_rows = base  # the oracle checks read the rows the index holds
     rng = np.random.default_rng(9)
     reqs, off = [], 0
     while off < len(q):
@@ -116,3 +117,75 @@ def test_memory_admission_and_errors(served):
     ids, _, _ = srv.wait(t)
     assert (ids < ix.n).all()
     srv.close()
+
+
+def _check_against_oracle(base, rows, ids, d, k, gt, gtd):
+    """exact distances (the oracle's) per returned id, sorted, distinct;
+    returns the hits counted as the reference's recall does"""
+    import oracle as O
+    from nbpg_amd.autotune import recall_with_ties
+    hits = 0.0
+    for j in range(len(rows)):
+        assert (np.diff(d[j, :k]) >= 0).all()
+        assert len(set(ids[j, :k].tolist())) == k
+        ref = np.array([O.distance(O.U8, base[i], rows[j]) for i in ids[j, :k]], np.float32)
+        assert (ref == d[j, :k]).all(), j
+        hits += recall_with_ties(ids[j:j + 1, :k], gt[j:j + 1], gtd[j:j + 1], k) * k
+    return hits
+
+
+def test_uneven_and_served_match_oracle(served):
+    """nbpg_search_uneven (mixed L and k in one launch) and the server's
+    results against the oracle: each returned id's exact distance, sorted
+    distinct ids, and recall@k against the oracle's exact ground truth."""
+    N, ix, reqs = served
+    import torch
+    import oracle as O
+    base = ix.host_rows
+    sub = reqs[:40]
+    rows = np.concatenate([r[0] for r in sub])
+    Ls = np.concatenate([[r[1]] * len(r[0]) for r in sub]).astype(np.int32)
+    ks = np.concatenate([[r[2]] * len(r[0]) for r in sub]).astype(np.int32)
+    gt, gtd = O.compgt(base, rows, 100, O.U8, tie_mode=1)
+    stride = ix.row_stride
+    qpad = np.zeros((len(rows), stride), np.uint8)
+    qpad[:, :rows.shape[1]] = rows
+    ids, d, _ = N.search_uneven(ix, N.to_device(qpad), torch.from_numpy(Ls).cuda(),
+                                torch.from_numpy(ks).cuda(), 256, 100, fast=FAST)
+    ids, d = ids.cpu().numpy().view(np.uint32), d.cpu().numpy()
+    hits, want = 0.0, 0
+    for j in range(len(rows)):
+        k = int(ks[j])
+        hits += _check_against_oracle(base, rows[j:j + 1], ids[j:j + 1], d[j:j + 1], k,
+                                      gt[j:j + 1], gtd[j:j + 1])
+        want += k
+    assert hits / want >= 0.95, hits / want
+    srv = N.Server(ix, slots=2, max_batch_queries=128, fast=FAST)
+    tickets = [srv.submit(r, L, k) for r, L, k in sub]
+    off, hits = 0, 0.0
+    for (r, L, k), t in zip(sub, tickets):
+        si, sd, _ = srv.wait(t)
+        hits += _check_against_oracle(base, r, si.view(np.uint32), sd, k,
+                                      gt[off:off + len(r)], gtd[off:off + len(r)])
+        off += len(r)
+    srv.close()
+    assert hits / want >= 0.95, hits / want
+
+
+def test_uneven_L_capped_at_max_L(served):
+    """a per-query L above max_L runs as max_L on every kernel shape"""
+    N, ix, reqs = served
+    import torch
+    rows = reqs[0][0][:16]
+    stride = ix.row_stride
+    qpad = np.zeros((len(rows), stride), np.uint8)
+    qpad[:, :rows.shape[1]] = rows
+    qd = N.to_device(qpad)
+    ks = torch.full((len(rows),), 10, dtype=torch.int32, device="cuda")
+    for max_L in (48, 200, 300):
+        hi = torch.full((len(rows),), 1000, dtype=torch.int32, device="cuda")
+        eq = torch.full((len(rows),), max_L, dtype=torch.int32, device="cuda")
+        a = N.search_uneven(ix, qd, hi, ks, max_L, 10, fast=FAST)
+        b = N.search_uneven(ix, qd, eq, ks, max_L, 10, fast=FAST)
+        assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), max_L
+        assert torch.equal(a[2], b[2]), max_L
