@@ -193,6 +193,28 @@ int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq
                        const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
                        const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
                        uint32_t *d_out_stats, void *stream);
+/* nbpg_search_rerank without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, modes and results as nbpg_search_rerank, but
+ * the scratch it needs -- FAST mode's nq x L candidate ids, reference mode's
+ * per-workgroup search state when it does not fit in LDS (L >= 128 or so;
+ * the grid then shrinks to what the scratch holds) -- lives in one fixed
+ * array of NBPG_NOALLOC_SCRATCH_BYTES per GPU that the library holds until
+ * the process ends, and no hipMalloc / hipFree or allocating library call is
+ * made.  Limit: FAST mode needs nq * L * 4 <= NBPG_NOALLOC_SCRATCH_BYTES
+ * (else NBPG_EINVAL); reference mode needs one query's state to fit (else
+ * NBPG_EUNSUP).  Every call on a GPU shares that GPU's array: calls that
+ * overlap in time on one GPU race (issue them on one stream). */
+#define NBPG_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int nbpg_search_rerank_noalloc(const nbpg_index *idx, const void *d_queries, uint64_t nq,
+                               uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                               uint32_t raw_dims, const void *d_raw_queries,
+                               uint64_t raw_query_stride, const uint32_t *h_start_points,
+                               uint32_t sp_num, const nbpg_query_params *qp,
+                               uint32_t rerank_factor, int mode, const nbpg_fast_config *fast,
+                               uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
+                               void *stream);
 /* FAST mode with per-query beam width and k (uneven batches); d_L, d_k are
  * nq u32 each on the device, outputs have stride out_k per query.  A query's
  * L is capped at max_L (and raised to 1), its k at min(L, out_k), by every

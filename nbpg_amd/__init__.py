@@ -84,6 +84,8 @@ _SIGS = {
     "nbpg_l2_pairs": (I32, [I32, VP, VP, U64, U32, VP, VP]),
     "nbpg_search_rerank": (I32, [VP, VP, U64, U64, I32, VP, U32, VP, U64, VP, U32,
                                  C.POINTER(QueryParams), U32, I32, VP, VP, VP, VP, VP]),
+    "nbpg_search_rerank_noalloc": (I32, [VP, VP, U64, U64, I32, VP, U32, VP, U64, VP, U32,
+                                         C.POINTER(QueryParams), U32, I32, VP, VP, VP, VP, VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
                           C.POINTER(FastConfig), VP, VP, VP, VP]),
     "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, U32, C.POINTER(FastConfig),
@@ -257,12 +259,14 @@ def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_lim
 
 def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100,
                   mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None, entry=0,
-                  start_points=None, fast=None, raw_dtype=None):
+                  start_points=None, fast=None, raw_dtype=None, noalloc=False):
     """Search over the index's quantized rows with full-precision rows for the
     final distances (beam_search_rerank): rerank_factor > 0 re-scores the
     frontier head, 0 is the filtering mode (SEARCH_REF only).  raw_base_dev:
     n x raw_dims (float32 / uint8 / int8 tensor), raw_q_dev: nq x raw_dims.
     raw_dtype overrides the type taken from the tensor (U4X2 / I4X2 rows).
+    noalloc: call nbpg_search_rerank_noalloc (scratch in the library's fixed
+    per-GPU array, no device allocation by the library).
     Returns (ids, dists, stats), nq x k."""
     torch = _torch()
     nq = q_dev.shape[0]
@@ -276,12 +280,13 @@ def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100
     fc = FastConfig(*fast) if fast is not None else None
     raw_dims = raw_base_dev.shape[1]
     rdt = _x_dtype(raw_base_dev) if raw_dtype is None else raw_dtype
-    _check(lib().nbpg_search_rerank(index.h, _ptr(q_dev), nq, qstride, rdt,
-                                    _ptr(raw_base_dev), raw_dims, _ptr(raw_q_dev),
-                                    raw_q_dev.shape[1] * raw_q_dev.element_size(),
-                                    sp.ctypes.data, len(sp), C.byref(qp), rerank_factor, mode,
-                                    C.byref(fc) if fc is not None else None, _ptr(ids), _ptr(d),
-                                    _ptr(st), _stream()), "nbpg_search_rerank")
+    name = "nbpg_search_rerank_noalloc" if noalloc else "nbpg_search_rerank"
+    _check(getattr(lib(), name)(index.h, _ptr(q_dev), nq, qstride, rdt, _ptr(raw_base_dev),
+                                raw_dims, _ptr(raw_q_dev),
+                                raw_q_dev.shape[1] * raw_q_dev.element_size(), sp.ctypes.data,
+                                len(sp), C.byref(qp), rerank_factor, mode,
+                                C.byref(fc) if fc is not None else None, _ptr(ids), _ptr(d),
+                                _ptr(st), _stream()), name)
     return ids, d, st
 
 

@@ -13,6 +13,7 @@
 //   beam; W beam entries expanded per step; candidates worse than the beam's
 //   tail are dropped before the merge.
 #include <algorithm>
+#include <mutex>
 
 #include "ref_search.h"
 #include "search_args.h"
@@ -313,10 +314,14 @@ struct ExactRows {
     uint32_t rerank_factor; // 0: filtering mode
 };
 
+// fixed: caller-provided scratch (fixed_bytes) used instead of the index's
+// workspace when the per-query state does not fit in LDS; the grid shrinks
+// to what it holds
 static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
                       const uint32_t *sp, uint32_t sp_num, const nbpg_query_params *qp,
                       uint32_t out_k, const ExactRows *ex, uint32_t *d_ids, float *d_dists,
-                      uint32_t *d_stats, hipStream_t stream)
+                      uint32_t *d_stats, hipStream_t stream, uint8_t *fixed = nullptr,
+                      uint64_t fixed_bytes = 0)
 {
     SearchArgs a;
     memset(&a, 0, sizeof(a));
@@ -361,7 +366,14 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
     const bool use_lds = lo.total <= lds_budget;
     uint64_t grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (use_lds ? 32 : 8));
-    if (!use_lds) {
+    if (!use_lds && fixed) {
+        grid = std::min<uint64_t>(grid, fixed_bytes / lo.total);
+        if (grid == 0)
+            return set_error(NBPG_EUNSUP, "reference-mode state of %llu bytes per query exceeds "
+                             "the %llu-byte scratch", (unsigned long long)lo.total,
+                             (unsigned long long)fixed_bytes);
+        a.gscratch = fixed;
+    } else if (!use_lds) {
         NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), grid * lo.total));
         a.gscratch = (uint8_t *)ix->ws;
     }
@@ -482,6 +494,36 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     return NBPG_OK;
 }
 
+// Scratch of the allocation-free entry point (nbpg_search_rerank_noalloc):
+// one array per GPU, loaded with this code object and held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on
+// the host the name is only a stand-in); it passes the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device and kept.  Calls on one GPU share the array, so calls that overlap
+// in time on one GPU race.
+static __device__ __align__(256) uint8_t scratch[NBPG_NOALLOC_SCRATCH_BYTES];
+
+static int scratch_on_current_device(uint8_t **out)
+{
+    static std::mutex mu;
+    static uint8_t *addr[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return set_error(NBPG_EUNSUP, "device ordinal %d", dev);
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(NBPG_EHIP, "scratch address on device %d: %s", dev,
+                             hipGetErrorString(e));
+        addr[dev] = static_cast<uint8_t *>(p);
+    }
+    *out = addr[dev];
+    return NBPG_OK;
+}
+
 } // namespace nbpg
 
 using namespace nbpg;
@@ -537,13 +579,16 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                        d_out_dists, d_out_stats, ix->counter, as_stream(stream));
 }
 
-int nbpg_search_rerank(const nbpg_index *ix, const void *d_queries, uint64_t nq,
-                       uint64_t query_stride, int raw_dtype, const void *d_raw_base,
-                       uint32_t raw_dims, const void *d_raw_queries, uint64_t raw_query_stride,
-                       const uint32_t *h_start_points, uint32_t sp_num,
-                       const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
-                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
-                       uint32_t *d_out_stats, void *stream)
+// nbpg_search_rerank's work; fixed != NULL: every scratch buffer comes from
+// the fixed_bytes at `fixed` and nothing is allocated
+static int search_rerank_impl(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                              uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                              uint32_t raw_dims, const void *d_raw_queries,
+                              uint64_t raw_query_stride, const uint32_t *h_start_points,
+                              uint32_t sp_num, const nbpg_query_params *qp,
+                              uint32_t rerank_factor, int mode, const nbpg_fast_config *fast,
+                              uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
+                              void *stream, uint8_t *fixed, uint64_t fixed_bytes)
 {
     NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode,
                           qp && qp->k, d_out_ids));
@@ -563,18 +608,60 @@ int nbpg_search_rerank(const nbpg_index *ix, const void *d_queries, uint64_t nq,
         const ExactRows ex = {raw_dtype, d_raw_base, d_raw_queries, raw_dims, raw_query_stride,
                               rerank_factor};
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, qp->k, &ex,
-                          d_out_ids, d_out_dists, d_out_stats, s);
+                          d_out_ids, d_out_dists, d_out_stats, s, fixed, fixed_bytes);
     }
-    // FAST: the top L of each beam into the index's workspace, then re-scored
+    // FAST: the top L of each beam into scratch, then re-scored
     if (std::min<uint64_t>((uint64_t)qp->k * rerank_factor, qp->L) > RERANK_MAX)
         return set_error(NBPG_EUNSUP, "min(k * rerank_factor, L) > %d", RERANK_MAX);
-    NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), nq * qp->L * sizeof(uint32_t)));
-    uint32_t *cand = (uint32_t *)ix->ws;
+    const uint64_t cand_bytes = nq * qp->L * sizeof(uint32_t);
+    uint32_t *cand;
+    if (fixed) {
+        if (cand_bytes > fixed_bytes)
+            return set_error(NBPG_EINVAL, "nq * L = %llu candidates exceed the %llu-byte scratch",
+                             (unsigned long long)(nq * qp->L), (unsigned long long)fixed_bytes);
+        cand = reinterpret_cast<uint32_t *>(fixed);
+    } else {
+        NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), cand_bytes));
+        cand = (uint32_t *)ix->ws;
+    }
+    // every query writes all L of its slots (NBPG_SENT past its frontier),
+    // so the scratch needs no clearing between calls
     NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
                          sp_num, qp->L, qp->L, nullptr, nullptr, qp->L, qp->L, qp->limit, fast,
                          cand, nullptr, d_out_stats, ix->counter, s));
     return rerank_launch(raw_dtype, ix->mips, d_raw_base, ix->n, raw_dims, d_raw_queries, raw_query_stride,
                          nq, cand, qp->L, qp->L, qp->k, rerank_factor, d_out_ids, d_out_dists, s);
+}
+
+int nbpg_search_rerank(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                       uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                       uint32_t raw_dims, const void *d_raw_queries, uint64_t raw_query_stride,
+                       const uint32_t *h_start_points, uint32_t sp_num,
+                       const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
+                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                       uint32_t *d_out_stats, void *stream)
+{
+    return search_rerank_impl(ix, d_queries, nq, query_stride, raw_dtype, d_raw_base, raw_dims,
+                              d_raw_queries, raw_query_stride, h_start_points, sp_num, qp,
+                              rerank_factor, mode, fast, d_out_ids, d_out_dists, d_out_stats,
+                              stream, nullptr, 0);
+}
+
+int nbpg_search_rerank_noalloc(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                               uint64_t query_stride, int raw_dtype, const void *d_raw_base,
+                               uint32_t raw_dims, const void *d_raw_queries,
+                               uint64_t raw_query_stride, const uint32_t *h_start_points,
+                               uint32_t sp_num, const nbpg_query_params *qp,
+                               uint32_t rerank_factor, int mode, const nbpg_fast_config *fast,
+                               uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
+                               void *stream)
+{
+    uint8_t *ws = nullptr;
+    NBPG_TRY(scratch_on_current_device(&ws));
+    return search_rerank_impl(ix, d_queries, nq, query_stride, raw_dtype, d_raw_base, raw_dims,
+                              d_raw_queries, raw_query_stride, h_start_points, sp_num, qp,
+                              rerank_factor, mode, fast, d_out_ids, d_out_dists, d_out_stats,
+                              stream, ws, NBPG_NOALLOC_SCRATCH_BYTES);
 }
 
 int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
