@@ -492,9 +492,11 @@ def pack_cands(ids, dists):
 
 
 def build(index, R, L, alpha=1.2, passes=1, num_split=0, VL=140, min_alpha=1.0,
-          dynamic_prune=False, rand_sp=False, repair=True, top_degree=0, sp_seed=1, limit=0):
-    bp = BuildParams(R, L, VL, top_degree, num_split, passes, limit, min_alpha, alpha, 0.0, 0.0,
-                     int(dynamic_prune), int(rand_sp), int(repair), sp_seed)
+          dynamic_prune=False, rand_sp=False, repair=True, top_degree=0, sp_seed=1, limit=0,
+          min_cos_angle=0.0, max_cos_angle=0.0):
+    bp = BuildParams(R, L, VL, top_degree, num_split, passes, limit, min_alpha, alpha,
+                     min_cos_angle, max_cos_angle, int(dynamic_prune), int(rand_sp), int(repair),
+                     sp_seed)
     iso = U64(0)
     _check(lib().nbpg_build(index.h, C.byref(bp), C.byref(iso)), "nbpg_build")
     return iso.value

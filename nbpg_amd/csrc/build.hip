@@ -63,7 +63,7 @@ struct InsertArgs {
     uint64_t sp_seed, sp_ctr;
     uint32_t L, VL, H;
     uint64_t limit;
-    const float *alphas;
+    const float *alphas, *coss; // per-node schedules (PopularPrune), or NULL
     float alpha_f, cos_f;
     float *popular;
     int cal_popular;
@@ -177,9 +177,10 @@ __global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
         wave_sync();
         const uint32_t mm = wave_sort_candidates(cp, nm, index, lane);
         const float af = a.alphas ? a.alphas[index] : a.alpha_f;
+        const float cf = a.coss ? a.coss[index] : a.cos_f;
         uint32_t comps = 0;
         const uint32_t sel =
-            wave_prune<DT, NCH>(cp, mm, index, a.R, (double)af, (double)a.cos_f, a.vecs, a.stride,
+            wave_prune<DT, NCH>(cp, mm, index, a.R, (double)af, (double)cf, a.vecs, a.stride,
                                 a.norms, a.p_ids + j * a.R, a.p_d + j * a.R, comps, lane, a.mips);
         if (lane == 0)
             a.p_cnt[j] = sel;
@@ -364,7 +365,7 @@ struct BuildCtx {
     uint64_t *keyA = nullptr, *keyB = nullptr;
     void *cub_tmp = nullptr;
     size_t cub_bytes = 0;
-    float *popular = nullptr, *alphas = nullptr;
+    float *popular = nullptr, *alphas = nullptr, *coss = nullptr;
     uint8_t *gscratch = nullptr;
     uint64_t wave_bytes = 0;
     bool use_lds = true;
@@ -390,6 +391,7 @@ static void build_free(BuildCtx &c)
     hipFree(c.cub_tmp);
     hipFree(c.popular);
     hipFree(c.alphas);
+    hipFree(c.coss);
     hipFree(c.gscratch);
 }
 
@@ -422,28 +424,25 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     const uint64_t m = inserts.size();
     HIP_TRY(hipMemcpyAsync(c.d_inserts, inserts.data(), m * 4, hipMemcpyHostToDevice, c.s));
     float alpha_f = 1.0f, cos_f = (float)bp->min_cos_angle; // nsw.h:297-324
-    const float *alphas = nullptr;
+    const float *alphas = nullptr, *coss = nullptr;
     if (pop_mode == POP_FIXED_MIN) {
         alpha_f = (float)bp->max_alpha;
         cos_f = (float)bp->max_cos_angle;
     } else if (pop_mode == POP_PRUNE) {
         const float minA = (float)bp->min_alpha, maxA = (float)bp->max_alpha;
         const float minC = (float)bp->min_cos_angle, maxC = (float)bp->max_cos_angle;
-        std::vector<float> al(c.N);
-        float cof0 = 0.f;
+        // both thresholds interpolate on the node's popularity, in float
+        std::vector<float> al(c.N), co(c.N);
         for (uint64_t i = 0; i < c.N; ++i) {
             const float cof = (c.h_popular[i] - c.maxPop) / (c.minPop - c.maxPop);
             al[i] = minA + cof * (maxA - minA);
-            if (i == 0)
-                cof0 = cof;
+            co[i] = minC + cof * (maxC - minC);
         }
-        // the cos threshold follows the same schedule; all callers use one value
-        cos_f = minC + cof0 * (maxC - minC);
-        if (bp->max_cos_angle != bp->min_cos_angle)
-            return set_error(NBPG_EUNSUP, "per-node cos_angle schedule");
         HIP_TRY(hipMemcpyAsync(c.alphas, al.data(), c.N * 4, hipMemcpyHostToDevice, c.s));
+        HIP_TRY(hipMemcpyAsync(c.coss, co.data(), c.N * 4, hipMemcpyHostToDevice, c.s));
         HIP_TRY(hipStreamSynchronize(c.s));
         alphas = c.alphas;
+        coss = c.coss;
     }
     if (pop_mode == POP_CAL)
         HIP_TRY(hipMemsetAsync(c.popular, 0, c.N * 4, c.s));
@@ -468,6 +467,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     a.H = c.H;
     a.limit = bp->limit ? bp->limit : c.N;
     a.alphas = alphas;
+    a.coss = coss;
     a.alpha_f = alpha_f;
     a.cos_f = cos_f;
     a.popular = c.popular;
@@ -607,8 +607,6 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
     const uint32_t nsp_max = bp->num_split > 1 ? bp->num_split : 1;
     if (nsp_max > bp->L || (bp->num_split > 1 && bp->num_split > ix->n)) // query_params.h:206-210
         return set_error(NBPG_EINVAL, "start points %u > beam size %u", nsp_max, bp->L);
-    if (bp->dynamic_prune && bp->max_cos_angle != bp->min_cos_angle)
-        return set_error(NBPG_EUNSUP, "per-node cos_angle schedule");
     NBPG_TRY(ensure_init());
     BuildCtx c;
     c.ix = ix;
@@ -637,7 +635,8 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
               hipMalloc(&c.num_runs, 16) == hipSuccess && hipMalloc(&c.keyA, ne * 8) == hipSuccess &&
               hipMalloc(&c.keyB, ne * 8) == hipSuccess &&
               hipMalloc(&c.popular, c.N * 4) == hipSuccess &&
-              hipMalloc(&c.alphas, c.N * 4) == hipSuccess;
+              hipMalloc(&c.alphas, c.N * 4) == hipSuccess &&
+              hipMalloc(&c.coss, c.N * 4) == hipSuccess;
     if (!c.use_lds && ok) {
         c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 8);
         ok = hipMalloc(&c.gscratch, (uint64_t)c.ins_grid * c.wave_bytes) == hipSuccess;

@@ -288,7 +288,7 @@ def repair(rows, sp=0):
 
 def build_graph(base, dtype, R, L, alpha=1.2, passes=1, num_split=0, VL=140, min_alpha=1.0,
                 dynamic_prune=False, rand_sp=False, repair=True, top_degree=0, sp_seed=1,
-                rows=None, nthreads=0):
+                rows=None, nthreads=0, min_cos_angle=0.0, max_cos_angle=0.0):
     """build_vamana restated; returns reference-layout rows (n, R+1)."""
     base = np.ascontiguousarray(base)
     n = base.shape[0]
@@ -297,8 +297,8 @@ def build_graph(base, dtype, R, L, alpha=1.2, passes=1, num_split=0, VL=140, min
     rows = np.ascontiguousarray(rows, dtype=np.uint32)
     G = Graph(rows.ctypes.data, n, R)
     P = points(base, dtype)
-    bp = BuildParams(R, L, VL, top_degree, num_split, passes, n, min_alpha, alpha, 0.0, 0.0,
-                     int(dynamic_prune), int(rand_sp), int(repair), sp_seed)
+    bp = BuildParams(R, L, VL, top_degree, num_split, passes, n, min_alpha, alpha, min_cos_angle,
+                     max_cos_angle, int(dynamic_prune), int(rand_sp), int(repair), sp_seed)
     iso = C.c_uint64(0)
     rc = lib().nbpgo_build(C.byref(P), C.byref(G), C.byref(bp), C.byref(iso), nthreads)
     if rc != 0:

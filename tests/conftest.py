@@ -54,3 +54,12 @@ def gpu():
     import nbpg_amd as N
     N.init(0)
     return N
+
+
+@pytest.fixture(scope="session")
+def golden_branches():
+    """Angle prune, cosine / top_degree builds and limit / degree_limit / wide-L
+    searches recorded from the compiled reference."""
+    path = os.path.join(ROOT, "tests", "golden", "reference_branches.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
