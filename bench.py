@@ -266,13 +266,28 @@ def main():
     plain_qps = nq * args.plain_steps / pwall
     log("plain L=%d: %.0f QPS" % (L_plain, plain_qps))
 
-    # roofline of the fast kernel: algorithmic bytes per launch
-    vis = st_fast[:, 0].astype(np.float64)
-    cmps = st_fast[:, 1].astype(np.float64)
+    # roofline of the fast kernel: algorithmic bytes of one launch from each
+    # query's own counters -- expansions, distance computations and the
+    # adjacency entries its expanded nodes hold (NBPG_SEARCH_EDGES):
+    #   B_q = 4 * (visited + edges) + D * dist_cmps + D
+    # The seed rows' distances (every query starts with one per seed) hit
+    # the same L2-resident rows for every query; the headline figure leaves
+    # them out and the figure with them is reported beside it.
+    e_ids, _, st3 = N.search(ix, q_d, L_fast, 10, mode=N.SEARCH_FAST, fast=fast,
+                             start_points=seeds, edges=True)
+    if not np.array_equal(e_ids.cpu().numpy(), run_fast(L_fast)[0].cpu().numpy()):
+        raise SystemExit("FAST results differ between two calls")
+    st3 = st3.cpu().numpy().astype(np.float64)
+    vis, cmps, edges = st3[:, 0], st3[:, 1], st3[:, 2]
     dbytes = base.shape[1]
-    alg_bytes = float((vis * 4 * (1 + avg_deg)).sum() + (cmps * dbytes).sum() + nq * dbytes)
+    nseed = len(seeds) if seeds is not None else 1
+    alg_with_seeds = float(4 * (vis + edges).sum() + (cmps * dbytes).sum() + nq * dbytes)
+    alg_bytes = alg_with_seeds - float(nq * nseed * dbytes)
     kernel_ms = float(per.mean())
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    achieved_seeds = alg_with_seeds / (kernel_ms * 1e-3) / 1e9
+    log("roofline: %.1f MB per launch without the seed rows (%.1f MB with), %.0f / %.0f GB/s"
+        % (alg_bytes / 1e6, alg_with_seeds / 1e6, achieved, achieved_seeds))
 
     cpu = None
     if not args.no_cpu:
@@ -302,7 +317,14 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(base.shape[0], nq, base.shape[1], L_fast)},
+                     "traffic": measured_traffic(base.shape[0], nq, base.shape[1], L_fast, fast),
+                     "algorithmic_bytes": round(alg_bytes), "kernel_ms": round(kernel_ms, 5),
+                     "bytes": "sum over queries of 4*(visited + adjacency entries of the "
+                              "expanded nodes) + 128*(dist_cmps - seed rows) + 128, from the "
+                              "kernel's own per-query counters",
+                     "with_seed_rows": {"algorithmic_bytes": round(alg_with_seeds),
+                                        "achieved": round(achieved_seeds, 1),
+                                        "frac": round(achieved_seeds / PEAK_HBM_GBS, 4)}},
         "cpu_baseline": cpu,
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
@@ -322,6 +344,8 @@ def main():
         "groundtruth_refrule_queries_differing": gt_ref_differs,
         "mean_visited": round(float(vis.mean()), 1),
         "mean_dist_cmps": round(float(cmps.mean()), 1),
+        "mean_expanded_degree": round(float(edges.sum() / max(vis.sum(), 1.0)), 2),
+        "graph_avg_degree": round(avg_deg, 2),
         "autotune": [{"config": list(r["config"]), "L": r["L"], "qps": round(r["qps"], 1)}
                      for r in (tune_table or [])],
     }
@@ -331,32 +355,47 @@ def main():
 GTD = None
 
 
-def measured_traffic(n, nq, dims, L):
+def measured_traffic(n, nq, dims, L, fast):
     """HBM bytes per FAST launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy), or
-    None when this run's workload differs from the profiled one."""
+    WRITE_SIZE summaries of this workload (calibrated against a 1 GiB copy),
+    one per profiled launch shape; None when this run's workload or launch
+    shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "search_reg_kernel_1m_u8_L16.json")
     try:
         with open(path) as f:
-            p = json.load(f)
+            prof = json.load(f)
     except OSError:
         return None
-    if (p["n"], p["nq"], p["dims"], p["L"]) != (n, nq, dims, L):
-        return None
-    return p["traffic_bytes_per_launch"]
+    for p in prof["runs"]:
+        if (p["n"], p["nq"], p["dims"], p["L"], list(p["fast_config"])) == \
+                (n, nq, dims, L, list(fast)):
+            return p["traffic_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(base, q, rows, gt, L, seconds):
     """The reference's qsearchAll on the host cores, bounded sample."""
+    # threads pinned to cores, one per core: read by the host OpenMP runtime
+    # when the reference library loads (torch carries its own runtime)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     try:
         import oracle as O
     except Exception as e:  # pragma: no cover
         log("cpu baseline unavailable:", e)
         return None
-    R = O.ref()
+    # the reference's build flags (-Ofast -funroll-loops ..., at the host's
+    # ISA level); the parity build when the timing build is missing
+    R, isa = O.ref_fast()
+    flags = "-Ofast -fopenmp -fopenmp-simd -funroll-loops -falign-jumps=64 -mcx16 " \
+            "-march=x86-64-%s" % isa
+    if R is None:
+        R = O.ref()
+        flags = "-O3 -fopenmp -ffp-contract=off (parity build)"
     kind = "reference"
     if R is None:
         kind = "port"
+        flags = "oracle -O2 -fopenmp"
     k = 10
     if kind == "reference":
         h = R.nbpgr_open(base.ctypes.data, base.shape[0], base.shape[1], rows.ctypes.data,
@@ -390,8 +429,11 @@ def cpu_baseline(base, q, rows, gt, L, seconds):
     rec = recall_with_ties(ids, gt[:nq], GTD[:nq], 10)
     log("cpu %s: %d x %d queries in %.2fs on %d threads, recall %.4f" %
         (kind, reps, nq, total, threads, rec))
+    model, _ = O.host_cpu()
     return {"value": round(reps * nq / total, 1), "unit": "queries/s", "cores": int(threads),
-            "kind": kind, "recall@10": round(rec, 4),
+            "kind": kind, "recall@10": round(rec, 4), "flags": flags, "cpu": model,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES"),
             "sample": "%d passes over the %d queries (%.1f s), qsearchAll at L=%d, k=10, cut 1.35, "
                       "same graph" % (reps, nq, total, L)}
 

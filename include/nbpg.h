@@ -52,6 +52,10 @@ extern "C" {
  * output stride L) instead of its first k; the reference mode still cuts
  * with k.  This is the candidate list the rerank step reads. */
 #define NBPG_SEARCH_FRONTIER 0x10
+/* OR into nbpg_search's NBPG_SEARCH_FAST mode: stats rows get a third u32,
+ * the adjacency entries the query's expanded nodes hold (the sum of their
+ * degrees), for byte accounting; d_out_stats then holds nq*3 u32. */
+#define NBPG_SEARCH_EDGES 0x20
 
 /* ---- CompGT modes ---- */
 #define NBPG_GT_PLAIN 0 /* VALU dot4, one query tile per workgroup */

@@ -20,7 +20,7 @@ CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
 # PCACompress picks the code type from the output extension (PCACompress.cpp:25-37)
 CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
                ".i4bin": CODE_I4}
-SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER = 0, 1, 0x10
+SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER, SEARCH_EDGES = 0, 1, 0x10, 0x20
 METRIC_L2, METRIC_MIPS = 0, 1
 GT_PLAIN, GT_MFMA, GT_REFTIES, GT_MIPS = 0, 1, 0x100, 0x200
 SENT = 0xFFFFFFFF
@@ -231,10 +231,12 @@ def l2_batch(index, q_dev, ids_dev):
 
 
 def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None,
-           entry=0, start_points=None, fast=None, stats=True, out=None, frontier=False):
+           entry=0, start_points=None, fast=None, stats=True, out=None, frontier=False,
+           edges=False):
     """Beam search for every row of q_dev (uint8 tensor, nq x stride bytes).
     frontier=True returns each query's whole frontier (nq x L) instead of
-    its first k (the reference mode still cuts with k)."""
+    its first k (the reference mode still cuts with k).  edges=True (FAST
+    mode): stats rows are {visited, dist_cmps, adjacency entries read}."""
     torch = _torch()
     nq = q_dev.shape[0]
     qstride = q_dev.shape[1] * q_dev.element_size()
@@ -244,10 +246,13 @@ def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_lim
     width = L if frontier else k
     if frontier:
         mode |= SEARCH_FRONTIER
+    if edges:
+        mode |= SEARCH_EDGES
     if out is None:
         ids = torch.empty((nq, width), dtype=torch.int32, device="cuda")
         d = torch.empty((nq, width), dtype=torch.float32, device="cuda")
-        st = torch.empty((nq, 2), dtype=torch.int32, device="cuda") if stats else None
+        st = torch.empty((nq, 3 if edges else 2), dtype=torch.int32, device="cuda") \
+            if stats else None
     else:
         ids, d, st = out
     fc = FastConfig(*fast) if fast is not None else None

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
                 filt[fast_slot(e, a.flog2)] = e;
             }
         }
-        uint32_t bs = 1, nexp = 0, cmps = 1;
+        uint32_t bs = 1, nexp = 0, cmps = 1, edges = 0;
         wave_sync();
         while (nexp < a.limit) {
             // pick up to W unexpanded entries, best first
@@ -205,6 +205,7 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
 #pragma unroll
             for (int t = 0; t < IPL; ++t) {
                 bool f = ids[t] != NBPG_SENT;
+                edges += (uint32_t)__popcll(ballot(f));
                 if (f) {
                     const uint32_t s = fast_slot(ids[t], a.flog2);
                     f = filt[s] != ids[t];
@@ -293,8 +294,10 @@ __global__ __launch_bounds__(1024) void search_fast_kernel(SearchArgs a)
             }
         }
         if (a.out_stats && lane == 0) {
-            a.out_stats[2ull * qi] = nexp;
-            a.out_stats[2ull * qi + 1] = cmps;
+            a.out_stats[(uint64_t)a.stats_w * qi] = nexp;
+            a.out_stats[(uint64_t)a.stats_w * qi + 1] = cmps;
+            if (a.stats_w == 3)
+                a.out_stats[3ull * qi + 2] = edges;
         }
         wave_sync();
     }
@@ -396,7 +399,7 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                 uint32_t entry, const uint32_t *seeds, uint32_t nseeds, uint32_t L, uint32_t k,
                 const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
                 uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
-                uint32_t *d_stats, uint32_t *counter, hipStream_t stream)
+                uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w)
 {
     nbpg_fast_config c = {2, 1, 11, 0, 0};
     if (cfg)
@@ -459,6 +462,7 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     a.out_ids = d_ids;
     a.out_dists = d_dists;
     a.out_stats = d_stats;
+    a.stats_w = stats_w;
     const uint64_t lds = (uint64_t)a.wave_bytes * c.waves_per_group;
     if (lds > (uint64_t)g_dev.lds_max)
         return set_error(NBPG_EUNSUP, "fast search needs %llu B of LDS per group",
@@ -565,7 +569,10 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                 uint32_t *d_out_stats, void *stream)
 {
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
-    const int base = mode & ~NBPG_SEARCH_FRONTIER;
+    const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
+    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES);
+    if (edges && base != NBPG_SEARCH_FAST)
+        return set_error(NBPG_EINVAL, "NBPG_SEARCH_EDGES counts are kept by the FAST kernels");
     NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, base,
                           frontier || qp->k, d_out_ids));
     if (nq == 0)
@@ -576,7 +583,7 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                           nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
     return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points, sp_num,
                        qp->L, out_k, nullptr, nullptr, qp->L, out_k, qp->limit, fast, d_out_ids,
-                       d_out_dists, d_out_stats, ix->counter, as_stream(stream));
+                       d_out_dists, d_out_stats, ix->counter, as_stream(stream), edges ? 3 : 2);
 }
 
 // nbpg_search_rerank's work; fixed != NULL: every scratch buffer comes from

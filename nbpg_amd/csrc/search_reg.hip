@@ -154,7 +154,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             b[0] = (lane < ns && lane < L) ? (((uint64_t)hi << 32) | lo) : EMPTY;
         }
         uint32_t cmps = ns;
-        uint32_t nexp = 0;
+        uint32_t nexp = 0, edges = 0;
         wave_sync();
         while (nexp < a.limit) {
             // the W best unexpanded entries
@@ -196,6 +196,7 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
 #pragma unroll
             for (int t = 0; t < IPL; ++t) {
                 bool f = ids[t] != NBPG_SENT;
+                edges += (uint32_t)__popcll(ballot(f));
                 if (f) {
                     const uint32_t sl = fast_slot(ids[t], a.flog2);
                     f = filt[sl] != ids[t];
@@ -249,8 +250,10 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             }
         }
         if (a.out_stats && lane == 0) {
-            a.out_stats[2ull * qi] = nexp;
-            a.out_stats[2ull * qi + 1] = cmps;
+            a.out_stats[(uint64_t)a.stats_w * qi] = nexp;
+            a.out_stats[(uint64_t)a.stats_w * qi + 1] = cmps;
+            if (a.stats_w == 3)
+                a.out_stats[3ull * qi + 2] = edges;
         }
         wave_sync();
     }
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
         }
         wave_sync();
         score_list(ns);
-        uint32_t nexp = 0, cmps = ns;
+        uint32_t nexp = 0, cmps = ns, edges = 0;
         wave_sync();
         while (nexp < a.limit) {
             uint32_t got = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -448,6 +451,7 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
 #pragma unroll
             for (int t = 0; t < IPL; ++t) {
                 bool f = ids[t] != NBPG_SENT;
+                edges += (uint32_t)__popcll(ballot(f));
                 if (f) {
                     const uint32_t sl = fast_slot(ids[t], a.flog2);
                     f = filt[sl] != ids[t];
@@ -477,8 +481,10 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
             }
         }
         if (a.out_stats && lane == 0) {
-            a.out_stats[2ull * qi] = nexp;
-            a.out_stats[2ull * qi + 1] = cmps;
+            a.out_stats[(uint64_t)a.stats_w * qi] = nexp;
+            a.out_stats[(uint64_t)a.stats_w * qi + 1] = cmps;
+            if (a.stats_w == 3)
+                a.out_stats[3ull * qi + 2] = edges;
         }
         wave_sync();
     }

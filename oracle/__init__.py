@@ -112,58 +112,100 @@ def ref_available():
     return os.path.exists(os.path.join(HERE, "_ref", "libnbpg_ref.so"))
 
 
+def host_cpu():
+    """(model name, x86-64 ISA level 'v4' / 'v3' / None) of this host, from
+    /proc/cpuinfo."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    v3 = {"avx2", "fma", "bmi1", "bmi2", "movbe", "f16c", "lzcnt" if "lzcnt" in flags else "abm"}
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    level = "v4" if v3 <= flags and v4 <= flags else ("v3" if v3 <= flags else None)
+    return model, level
+
+
+_ref_fast = None
+
+
+def ref_fast():
+    """The reference's code built with its own optimisation flags
+    (-Ofast -funroll-loops ..., oracle/_ref/libnbpg_ref_fast_<isa>.so) for
+    the ISA level this host supports: (library, ISA level), or (None, None).
+    For timing only: -Ofast may reorder its float paths, so parity checks
+    use ref()."""
+    global _ref_fast
+    if _ref_fast is None:
+        _, level = host_cpu()
+        path = os.path.join(HERE, "_ref", "libnbpg_ref_fast_%s.so" % level) if level else None
+        if path is None or not os.path.exists(path):
+            return None, None
+        _ref_fast = (_bind_ref(C.CDLL(path)), level)
+    return _ref_fast
+
+
 def ref():
     """The reference's own code (oracle/_ref), or None when not built."""
     global _ref
     if _ref is None and ref_available():
-        R = C.CDLL(os.path.join(HERE, "_ref", "libnbpg_ref.so"))
-        R.nbpgr_distance.restype = C.c_float
-        R.nbpgr_distance.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32]
-        R.nbpgr_hash64_2.restype = C.c_uint64
-        R.nbpgr_hash64_2.argtypes = [C.c_uint64]
-        R.nbpgr_hash_filter_size.restype = C.c_uint32
-        R.nbpgr_hash_filter_size.argtypes = [C.c_uint32]
-        R.nbpgr_search.restype = C.c_int
-        R.nbpgr_search.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
-                                   C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
-                                   C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p,
-                                   C.c_void_p]
-        R.nbpgr_search_exact.restype = C.c_int
-        R.nbpgr_search_exact.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32,
-                                         C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
-                                         C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
-                                         C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
-                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
-                                         C.c_void_p]
-        R.nbpgr_repair.restype = C.c_uint64
-        R.nbpgr_repair.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32]
-        R.nbpgr_prune.restype = C.c_uint64
-        R.nbpgr_prune.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
-                                  C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_double]
-        R.nbpgr_prune_batch.restype = C.c_double
-        R.nbpgr_prune_batch.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
-                                        C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint32,
-                                        C.c_double, C.c_double, C.c_void_p]
-        R.nbpgr_shuffle.restype = None
-        R.nbpgr_shuffle.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32]
-        R.nbpgr_matmul.restype = None
-        R.nbpgr_matmul.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
-                                   C.c_uint32, C.c_void_p]
-        R.nbpgr_compress.restype = None
-        R.nbpgr_compress.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint32,
-                                     C.c_void_p]
-        R.nbpgr_open.restype = C.c_void_p
-        R.nbpgr_open.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]
-        R.nbpgr_timed_search.restype = C.c_double
-        R.nbpgr_timed_search.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
-                                         C.c_uint32, C.c_double, C.c_uint64, C.c_uint32,
-                                         C.c_uint32, C.c_void_p, C.c_void_p]
-        R.nbpgr_close.restype = None
-        R.nbpgr_close.argtypes = [C.c_void_p]
-        R.nbpgr_threads.restype = C.c_int
-        R.nbpgr_threads.argtypes = []
-        _ref = R
+        _ref = _bind_ref(C.CDLL(os.path.join(HERE, "_ref", "libnbpg_ref.so")))
     return _ref
+
+
+def _bind_ref(R):
+    """argument types of the reference shim's entry points"""
+    R.nbpgr_distance.restype = C.c_float
+    R.nbpgr_distance.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32]
+    R.nbpgr_hash64_2.restype = C.c_uint64
+    R.nbpgr_hash64_2.argtypes = [C.c_uint64]
+    R.nbpgr_hash_filter_size.restype = C.c_uint32
+    R.nbpgr_hash_filter_size.argtypes = [C.c_uint32]
+    R.nbpgr_search.restype = C.c_int
+    R.nbpgr_search.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                               C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                               C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p,
+                               C.c_void_p]
+    R.nbpgr_search_exact.restype = C.c_int
+    R.nbpgr_search_exact.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32,
+                                     C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                     C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
+                                     C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p]
+    R.nbpgr_repair.restype = C.c_uint64
+    R.nbpgr_repair.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32]
+    R.nbpgr_prune.restype = C.c_uint64
+    R.nbpgr_prune.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                              C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_double]
+    R.nbpgr_prune_batch.restype = C.c_double
+    R.nbpgr_prune_batch.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                                    C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint32,
+                                    C.c_double, C.c_double, C.c_void_p]
+    R.nbpgr_shuffle.restype = None
+    R.nbpgr_shuffle.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32]
+    R.nbpgr_matmul.restype = None
+    R.nbpgr_matmul.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
+                               C.c_uint32, C.c_void_p]
+    R.nbpgr_compress.restype = None
+    R.nbpgr_compress.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_uint32,
+                                 C.c_void_p]
+    R.nbpgr_open.restype = C.c_void_p
+    R.nbpgr_open.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]
+    R.nbpgr_timed_search.restype = C.c_double
+    R.nbpgr_timed_search.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                     C.c_uint32, C.c_double, C.c_uint64, C.c_uint32,
+                                     C.c_uint32, C.c_void_p, C.c_void_p]
+    R.nbpgr_close.restype = None
+    R.nbpgr_close.argtypes = [C.c_void_p]
+    R.nbpgr_threads.restype = C.c_int
+    R.nbpgr_threads.argtypes = []
+    return R
 
 
 def _p(a):

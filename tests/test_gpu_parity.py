@@ -556,3 +556,35 @@ def test_search_fast_other_types(gpu, gen, cfg):
     gt, _ = O.compgt(base, q, 100, dt, tie_mode=1)
     assert O.recall(ids, gt, base, q, dt, 10) >= 0.9, (gen, cfg)
     ix.close()
+
+
+@pytest.mark.parametrize("cfg", [(2, 1, 10, 0, 0), (2, 2, 11, 0, 1), (2, 1, 11, 0, 0)])
+def test_fast_edge_counts(gpu, cfg):
+    """NBPG_SEARCH_EDGES: the third stats word is the adjacency entries of
+    the expanded nodes.  On graphs where every node has the same degree d it
+    equals visited * d; ids, distances and the first two words are those of
+    the plain call.  L = 300 takes the LDS-beam kernel."""
+    N = gpu
+    rng = np.random.default_rng(31)
+    n, R = 3000, 24
+    base = D.mixture_u8(n, 64, 20, 25.0, seed=5)
+    q = D.mixture_u8(50, 64, 20, 25.0, seed=6)
+    for d in (R, 7):
+        rows = np.zeros((n, R + 1), np.uint32)
+        rows[:, 0] = d
+        for i in range(n):
+            nb = rng.choice(n - 1, d, replace=False)
+            rows[i, 1:1 + d] = nb + (nb >= i)
+        ix = make_index(N, base, N.U8, R, rows)
+        qd = dev_rows(N, q, ix.row_stride)
+        for L in (32, 300):
+            a = N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, fast=cfg)
+            b = N.search(ix, qd, L, 10, mode=N.SEARCH_FAST, fast=cfg, edges=True)
+            assert (a[0].cpu() == b[0].cpu()).all() and (a[1].cpu() == b[1].cpu()).all()
+            sa, sb = a[2].cpu().numpy(), b[2].cpu().numpy()
+            assert sb.shape == (len(q), 3)
+            assert (sa == sb[:, :2]).all(), (d, L)
+            assert (sb[:, 2] == sb[:, 0] * d).all(), (d, L)
+        with pytest.raises(N.NbpgError):
+            N.search(ix, qd, 32, 10, mode=N.SEARCH_REF, edges=True)
+        ix.close()
