@@ -30,6 +30,11 @@ struct GtArgs {
     uint64_t row_stride; // bytes between base rows (dims, or a multiple to sample)
     uint64_t *partial;   // [nq][slices][k] keys
     uint32_t mips;       // inner-product metric
+    // NULL, or a device list of query numbers: list entry j is query
+    // qidx[j], its partial results go to entry j, and the list's length is
+    // *qcount, read on the device (so a launch can be sized for the most
+    // entries and enqueued before the list is known)
+    const uint32_t *qidx, *qcount;
 };
 
 // ascending bitonic sort of a[0:n2) by the whole workgroup
@@ -117,14 +122,19 @@ __global__ __launch_bounds__(GT_THREADS) void compgt_plain_kernel(GtArgs a)
     __shared__ uint32_t cnt[GT_QT], tnum[GT_QT];
     __shared__ uint64_t thr[GT_QT];
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
-    const uint64_t q0 = (uint64_t)blockIdx.x * GT_QT;
-    const uint32_t nqt = (uint32_t)min<uint64_t>(GT_QT, a.nq - q0);
+    const uint64_t nq = a.qidx ? *a.qcount : a.nq;
     const uint32_t words = a.dims / 4;
+    // query tiles: one per workgroup row, or a loop over a device list
+    for (uint64_t tile = blockIdx.x; tile * GT_QT < nq; tile += gridDim.x) {
+    const uint64_t q0 = tile * GT_QT;
+    const uint32_t nqt = (uint32_t)min<uint64_t>(GT_QT, nq - q0);
     for (uint32_t i = tid; i < GT_QT * NW; i += GT_THREADS) {
         const uint32_t q = i / NW, w = i % NW;
         uint32_t v = 0;
-        if (q < nqt && w < words)
-            v = reinterpret_cast<const uint32_t *>(a.queries + (q0 + q) * a.dims)[w];
+        if (q < nqt && w < words) {
+            const uint64_t qr = a.qidx ? a.qidx[q0 + q] : q0 + q;
+            v = reinterpret_cast<const uint32_t *>(a.queries + qr * a.dims)[w];
+        }
         qs[q][w] = v;
     }
     if (tid < GT_QT) {
@@ -184,39 +194,51 @@ __global__ __launch_bounds__(GT_THREADS) void compgt_plain_kernel(GtArgs a)
     for (uint32_t q = 0; q < nqt; ++q)
         for (uint32_t i = tid; i < a.k; i += GT_THREADS)
             a.partial[((q0 + q) * a.slices + blockIdx.y) * a.k + i] = i < tnum[q] ? top[q][i] : ~0ull;
+    __syncthreads(); // the next tile reuses the shared lists
+    }
 }
 
 // per query: sort the slices' candidates, emit k
+// qidx / qcount as in GtArgs: entry j of the list is written to query qidx[j]
 __global__ __launch_bounds__(256) void compgt_merge_kernel(const uint64_t *__restrict__ partial,
                                                           uint32_t slices, uint32_t k,
                                                           uint32_t *__restrict__ ids,
-                                                          float *__restrict__ dists)
+                                                          float *__restrict__ dists,
+                                                          uint64_t nq_launch,
+                                                          const uint32_t *__restrict__ qidx,
+                                                          const uint32_t *__restrict__ qcount)
 {
     extern __shared__ uint64_t all[];
-    const uint64_t q = blockIdx.x;
+    const uint64_t nq = qidx ? *qcount : nq_launch;
     const uint32_t m = slices * k;
     uint32_t n2 = 1;
     while (n2 < m)
         n2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
-        all[i] = i < m ? partial[q * m + i] : ~0ull;
-    __syncthreads();
-    block_bitonic(all, n2);
-    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
-        ids[q * k + i] = key_id(all[i]);
-        if (dists)
-            dists[q * k + i] = key_dist(all[i]);
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
+            all[i] = i < m ? partial[q * m + i] : ~0ull;
+        __syncthreads();
+        block_bitonic(all, n2);
+        const uint64_t o = qidx ? qidx[q] : q;
+        for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+            ids[o * k + i] = key_id(all[i]);
+            if (dists)
+                dists[o * k + i] = key_dist(all[i]);
+        }
+        __syncthreads();
     }
 }
 
 int compgt_merge_launch(const uint64_t *partial, uint64_t nq, uint32_t slices, uint32_t k,
-                        uint32_t *ids, float *dists, hipStream_t s)
+                        uint32_t *ids, float *dists, hipStream_t s, const uint32_t *qidx,
+                        const uint32_t *qcount)
 {
     uint32_t m = slices * k, n2 = 1;
     while (n2 < m)
         n2 <<= 1;
-    hipLaunchKernelGGL(compgt_merge_kernel, dim3((uint32_t)nq), dim3(256), n2 * 8, s, partial,
-                       slices, k, ids, dists);
+    const uint32_t grid = (uint32_t)(qidx ? std::min<uint64_t>(nq, 2048) : nq);
+    hipLaunchKernelGGL(compgt_merge_kernel, dim3(grid), dim3(256), n2 * 8, s, partial, slices, k,
+                       ids, dists, nq, qidx, qcount);
     return hipGetLastError() == hipSuccess ? NBPG_OK : set_error(NBPG_EHIP, "compgt merge launch");
 }
 
@@ -233,22 +255,38 @@ int compgt_refrule(int dtype, const uint8_t *base, uint64_t n, const uint8_t *qu
                    uint32_t dims, uint32_t k, bool use_mfma, uint32_t mips, uint32_t *ids,
                    float *dists, hipStream_t s);
 
-// plain scan + slice merge into ids / dists (dists may be NULL).  Base row r
-// is at base + r * row_stride and is reported as id r.
-int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
-                        const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
-                        uint32_t mips, uint32_t *ids, float *dists, hipStream_t s)
+// slices of the base per query tile for `tiles` tiles: enough workgroups to
+// fill the device, slices of at least 2048 rows, at most 4096 / k of them
+static uint64_t plain_slices(uint64_t tiles, uint64_t n, uint32_t k)
 {
-    const uint32_t nch = nch_for_bytes(dims);
-    const uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
-    // enough workgroups to fill the device, slices of at least 2048 rows
     uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / std::max<uint64_t>(1, tiles));
     slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 2048));
-    slices = std::max<uint64_t>(1, std::min<uint64_t>(slices, 4096 / k));
+    return std::max<uint64_t>(1, std::min<uint64_t>(slices, 4096 / k));
+}
+
+// bytes of the partial buffer compgt_plain_core needs for nq queries
+uint64_t compgt_plain_partial_bytes(uint64_t nq, uint64_t n, uint32_t k, bool listed)
+{
+    const uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
+    return nq * plain_slices(listed ? std::min<uint64_t>(tiles, 256) : tiles, n, k) * k * 8;
+}
+
+// plain scan + slice merge into ids / dists (dists may be NULL), with a
+// caller-provided partial buffer (compgt_plain_partial_bytes).  Base row r
+// is at base + r * row_stride and is reported as id r.  qidx / qcount: an
+// optional device list of at most nq query numbers (see GtArgs); the launch
+// is then a loop over at most 256 query tiles' worth of workgroups.
+int compgt_plain_core(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                      const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
+                      uint32_t mips, uint32_t *ids, float *dists, hipStream_t s,
+                      uint64_t *partial, const uint32_t *qidx, const uint32_t *qcount)
+{
+    const uint32_t nch = nch_for_bytes(dims);
+    uint64_t tiles = (nq + GT_QT - 1) / GT_QT;
+    if (qidx)
+        tiles = std::min<uint64_t>(tiles, 256);
+    const uint64_t slices = plain_slices(tiles, n, k);
     const uint64_t slice_rows = (n + slices - 1) / slices;
-    uint64_t *partial = nullptr;
-    if (hipMallocAsync((void **)&partial, nq * slices * k * 8, s) != hipSuccess)
-        return set_error(NBPG_ENOMEM, "compgt partial buffer");
     GtArgs a;
     a.base = base;
     a.queries = queries;
@@ -261,15 +299,29 @@ int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row
     a.row_stride = row_stride;
     a.partial = partial;
     a.mips = mips;
+    a.qidx = qidx;
+    a.qcount = qcount;
     int rc = dispatch_dt_nch(dtype, nch, [&](auto DT, auto NCH) {
         hipLaunchKernelGGL((compgt_plain_kernel<DT.value, NCH.value>),
                            dim3((uint32_t)tiles, (uint32_t)slices), dim3(GT_THREADS), 0, s, a);
     });
     if (rc == NBPG_OK && hipGetLastError() != hipSuccess)
         rc = set_error(NBPG_EHIP, "compgt launch");
-    if (rc == NBPG_OK) {
-        rc = compgt_merge_launch(partial, nq, (uint32_t)slices, k, ids, dists, s);
-    }
+    if (rc == NBPG_OK)
+        rc = compgt_merge_launch(partial, nq, (uint32_t)slices, k, ids, dists, s, qidx, qcount);
+    return rc;
+}
+
+int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                        const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
+                        uint32_t mips, uint32_t *ids, float *dists, hipStream_t s)
+{
+    uint64_t *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, compgt_plain_partial_bytes(nq, n, k, false), s) !=
+        hipSuccess)
+        return set_error(NBPG_ENOMEM, "compgt partial buffer");
+    const int rc = compgt_plain_core(dtype, base, n, row_stride, queries, nq, dims, k, mips, ids,
+                                     dists, s, partial, nullptr, nullptr);
     hipFreeAsync(partial, s);
     return rc;
 }

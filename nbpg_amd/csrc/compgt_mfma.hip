@@ -1,20 +1,35 @@
 // compgt_mfma.hip -- exact brute-force top-k on int8 matrix cores.
 //
 // Same result as compgt_plain (the exact (dist, id) top-k of every query,
-// what compute_groundtruth at CompGT.cpp:29-134 aims at), three passes:
-//   1. threshold: the plain scan over S base rows sampled at a fixed stride
-//      gives each query's k-th smallest distance there.  The k-th smallest
-//      distance of a subset is never below the global k-th smallest, so every
-//      true top-k row has a distance <= this threshold.
-//   2. scan: v_mfma_i32_16x16x64_i8 Gram tiles.  u8 rows enter as x - 128
-//      (x ^ 0x80 per byte); |x-q|^2 = |x'-q'|^2 = |x'|^2 + |q'|^2 - 2<x',q'>
-//      is exact in int32.  A wave keeps 16*QW queries' fragments in
-//      registers and streams 16-row base blocks; every row within its
-//      query's threshold is appended to that query's candidate list.
-//   3. select: per query, a bitonic sort of its candidates; a query whose
-//      list overflowed is recomputed by the plain kernel.
+// what compute_groundtruth at CompGT.cpp:29-134 aims at), in four passes
+// per chunk of queries, all on one stream with no host synchronisation and
+// no allocation once the per-device workspace has grown to the call's size:
+//   1. threshold: v_mfma_i32_16x16x64_i8 distance tiles over S base rows
+//      sampled at a fixed stride.  Each lane keeps, per query, the two
+//      smallest distances of the rows it sees (a quarter of each 16-row
+//      block); a small kernel takes the J-th smallest of all those values as
+//      the query's bound U.  They are distances of distinct rows, so at
+//      least J rows lie within U, and about J * stride rows of the base.
+//   2. scan: the same tiles over every base row; a row is kept when its
+//      distance is within U.  The row term enters the matrix core as the
+//      accumulator's start value, so the common case per 16 x 16 tile is
+//      one compare on the largest of a lane's four values and one
+//      wave-wide branch.  Kept rows go to a segment of their own per
+//      (query, base slice, lane), counted in a register: no atomics and no
+//      memory round trip in the scan.
+//   3. select: per query, its segments gathered and bitonic-sorted.  They
+//      hold every row within U, so when they hold at least k rows (and no
+//      segment overflowed) the first k are the exact top-k, ties at the
+//      k-th distance included.  Otherwise the query goes on a device list ...
+//   4. ... that the plain scan redoes (a launch sized for the whole chunk
+//      that reads the list's length on the device; empty almost always).
+// Rows enter as x - 128 (x ^ 0x80 per byte, u8) or x (i8):
+// |x-q|^2 = |x'|^2 + |q'|^2 - 2<x',q'>, exact in int32.  Inner product:
+// <x,q> = <x',q'> + A(x) + A(q) (mf_norms_kernel).
 #include <algorithm>
 #include <climits>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -23,21 +38,29 @@ namespace nbpg {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-#define MF_CAND 4096 // candidate slots per query
-#define MF_QW 2      // 16-query A tiles per wave (2: 106 VGPRs, 4 waves per SIMD)
-#define MF_WAVES 4
+#ifndef MF_QW
+#define MF_QW 4 // 16-query tiles per wavefront (B operand, registers)
+#endif
+#ifndef MF_WAVES
+#define MF_WAVES 8 // wavefronts per workgroup: 512 queries share each staged block
+#endif
+#ifndef MF_BLK
+#define MF_BLK 64 // base rows per block staged through LDS
+#endif
+#define MF_J 24       // bound = the J-th smallest of the per-lane top-2 sample values
+#define MF_QCHUNK 16384
+#define MF_MAX_TSLICES 64
 
-int compgt_plain_launch(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
-                        const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
-                        uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
-int compgt_tiles(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
-                 const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k, bool use_mfma,
-                 uint32_t mips, uint32_t *ids, float *dists, hipStream_t s);
+uint64_t compgt_plain_partial_bytes(uint64_t nq, uint64_t n, uint32_t k, bool listed);
+int compgt_plain_core(int dtype, const uint8_t *base, uint64_t n, uint64_t row_stride,
+                      const uint8_t *queries, uint64_t nq, uint32_t dims, uint32_t k,
+                      uint32_t mips, uint32_t *ids, float *dists, hipStream_t s,
+                      uint64_t *partial, const uint32_t *qidx, const uint32_t *qcount);
 
 // per row, with x' = x ^ 0x80 = x - 128 (u8) or x (i8), over dims bytes:
-// L2: |x'|^2.  Inner product: half of what turns <x',q'> back into <x,q>,
-// <x,q> = <x',q'> + A(x) + A(q) with A(x) = 128 sum(x') + 8192 dims (u8), 0 (i8)
-// rows of dims bytes (a multiple of 16, 16-byte aligned): 8 lanes per row,
+// L2: |x'|^2.  Inner product: A(x) = 128 sum(x') + 8192 dims (u8), 0 (i8),
+// the part of <x,q> = <x',q'> + A(x) + A(q) that belongs to the row.
+// Rows of dims bytes (a multiple of 16, 16-byte aligned): 8 lanes per row,
 // each summing 16-byte pieces 128 bytes apart, then an xor reduction over
 // the 8 (integer sums: the split is exact)
 __global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, uint32_t dims,
@@ -66,203 +89,365 @@ __global__ void mf_norms_kernel(const uint8_t *__restrict__ rows, uint64_t n, ui
     }
 }
 
-// threshold key: the sample's k-th distance with the largest id, so every
-// row at that distance or closer passes `key <= thr`
-__global__ void mf_thresholds_kernel(const float *__restrict__ d, uint64_t nq, uint32_t k,
-                                     uint64_t *__restrict__ thr, uint32_t *__restrict__ cnt,
-                                     uint32_t *__restrict__ ovf)
-{
-    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq)
-        return;
-    thr[q] = make_key(d[q * k + k - 1], 0xFFFFFFFFu);
-    cnt[q] = 0;
-    ovf[q] = 0;
-}
+struct MfArgs {
+    const uint8_t *base;
+    uint64_t rows;       // rows of this pass (sample rows, or the whole base)
+    uint64_t row_step;   // pass row i is base row i * row_step
+    const uint8_t *queries;
+    uint64_t nq;
+    uint32_t dims;
+    int is_u8;
+    const uint32_t *nx;  // per base row: |x'|^2, or A(x)
+    const uint32_t *nqv; // per query: |q'|^2, or A(q)
+    uint64_t slice_rows;
+    const int32_t *U;    // scan: per-query bound on the partial distance
+    uint64_t *cand;      // scan: segments of seg_cap keys per (query, slice, lane group)
+    uint32_t *segcnt;    // scan: rows kept per segment (beyond seg_cap: overflowed)
+    uint32_t seg_cap;
+    int32_t *tv;         // threshold pass: per query and slice, 4 lanes x 2 values
+};
 
-// rows[i] = src[idx[i]] (dims bytes each)
-__global__ void mf_gather_kernel(const uint8_t *__restrict__ src, const uint32_t *__restrict__ idx,
-                                 uint32_t m, uint32_t dims, uint8_t *__restrict__ rows)
+// Distances are carried as the partial distance e = d - |q'|^2 (L2:
+// |x'|^2 - 2c) or e = d + A(q) (inner product: -c - A(x)), an exact int32;
+// per query the order of e is the order of d.  Each block row's w holds
+// |x'|^2 or -A(x); a missing row has w = 2^30, beyond every bound.
+template <int KS, bool MIPS, bool THRESH>
+__global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(KS <= 2 ? 4 : 2)))
+void mf_scan_kernel(MfArgs a)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)m * dims)
-        return;
-    const uint64_t i = t / dims, b = t % dims;
-    rows[t] = src[(uint64_t)idx[i] * dims + b];
-}
-
-// out[idx[i]] = compact[i] (k results each)
-__global__ void mf_scatter_kernel(const uint32_t *__restrict__ cids, const float *__restrict__ cd,
-                                  const uint32_t *__restrict__ idx, uint32_t m, uint32_t k,
-                                  uint32_t *__restrict__ ids, float *__restrict__ dists)
-{
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)m * k)
-        return;
-    const uint64_t i = t / k, j = t % k, o = (uint64_t)idx[i] * k + j;
-    ids[o] = cids[t];
-    if (dists)
-        dists[o] = cd[t];
-}
-
-// 64-row blocks of the slice pass through LDS: the workgroup's 256 threads
-// load a block once (KS 16-byte pieces each, into registers while the
-// previous block is scored, then into the other LDS buffer) and every
-// wavefront reads its MFMA operands from there, so a base row is read from
-// memory once per 16 * MF_QW * MF_WAVES queries.
-#define MF_BLK 64
-
-template <int KS>
-__global__ __launch_bounds__(64 * MF_WAVES) void mf_scan_kernel(
-    const uint8_t *__restrict__ base, uint64_t n, const uint8_t *__restrict__ queries, uint64_t nq,
-    uint32_t dims, int is_u8, const uint32_t *__restrict__ nx, const uint32_t *__restrict__ nqv,
-    const uint64_t *__restrict__ thr, uint64_t slice_rows, uint32_t *__restrict__ cnt,
-    uint64_t *__restrict__ cand, uint32_t mips)
-{
-    constexpr uint32_t PITCH = 64 * KS + 16; // row pitch in LDS (16-byte pad against bank conflicts)
+    constexpr uint32_t PITCH = 64 * KS + 16; // LDS row pitch (16-byte pad against bank conflicts)
+    constexpr int32_t MISSING = 1 << 30;
     __shared__ __align__(16) uint8_t rows_lds[2][MF_BLK * PITCH];
-    __shared__ int32_t xs_lds[2][MF_BLK];
+    __shared__ __align__(16) int32_t w_lds[2][MF_BLK];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-    const uint32_t row16 = lane % 16, kq = lane / 16;
+    const uint32_t col = lane % 16, kq = lane / 16;
     const uint64_t q0 = ((uint64_t)blockIdx.x * MF_WAVES + wv) * (16 * MF_QW);
-    const bool active = q0 < nq; // inactive waves still load and synchronise
-    const uint32_t xmask = is_u8 ? 0x80808080u : 0u;
-    // query fragments: lane supplies 16 bytes of query row (l % 16) at k-chunk (l / 16)
-    v4i a[MF_QW][KS];
-    // the admission test in integers: with T the threshold distance (an
-    // exact integer), L2 admits xn - 2c <= T - qn, the inner product
-    // c + xn >= -T - qn; U holds the right-hand sides (never true for a
-    // missing query)
-    int32_t U[MF_QW][4];
-    uint32_t qn[MF_QW][4];
+    const bool active = q0 < a.nq; // inactive waves still stage blocks and synchronise
+    const uint32_t xmask = a.is_u8 ? 0x80808080u : 0u;
+    // D = rows x queries: the A operand is a 16-row block (lane: row `col`,
+    // 16 bytes at k-chunk kq), the B operand 16 queries (lane: query `col`,
+    // the same chunk), and lane l holds D[4 kq + r][col], r = 0..3
+    v4i qf[MF_QW][KS];
+    int32_t V[MF_QW], m1[MF_QW], m2[MF_QW];
+    uint32_t kept[MF_QW]; // scan: rows this lane kept per query tile
 #pragma unroll
     for (int t = 0; t < MF_QW; ++t) {
-        const uint64_t q = q0 + 16 * t + row16;
+        const uint64_t q = q0 + 16 * t + col;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const uint32_t off = 64 * s + 16 * kq;
             v4i v = {0, 0, 0, 0};
-            if (q < nq && off < dims) {
-                const uint4 u = *reinterpret_cast<const uint4 *>(queries + q * dims + off);
-                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
+            if (q < a.nq && off < a.dims) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(a.queries + q * a.dims + off);
+                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask),
+                        (int)(u.w ^ xmask)};
             }
-            a[t][s] = v;
+            qf[t][s] = v;
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint64_t qr = q0 + 16 * t + 4 * kq + r;
-            qn[t][r] = qr < nq ? nqv[qr] : 0u;
-            const int32_t T = qr < nq ? (int32_t)key_dist(thr[qr]) : 0;
-            U[t][r] = qr >= nq ? (mips ? INT_MAX : INT_MIN)
-                               : (mips ? -T - (int32_t)qn[t][r] : T - (int32_t)qn[t][r]);
-        }
+        // the scan's test on c' (below): c' >= V, V = -floor(U / 2) (L2) or
+        // -U (inner product); never true for a missing query
+        const int32_t u = q < a.nq ? a.U[q] : 0;
+        V[t] = THRESH || q >= a.nq ? INT_MAX : (MIPS ? -u : -(u >> 1));
+        m1[t] = INT_MAX;
+        m2[t] = INT_MAX;
+        kept[t] = 0;
     }
-    const uint64_t r0 = (uint64_t)blockIdx.y * slice_rows;
-    const uint64_t r1 = min(n, r0 + slice_rows);
-    // this thread's pieces of a block: piece p = tid + 256 j is row p / (4 KS),
-    // 16-byte chunk p % (4 KS)
-    v4i stage[KS];
-    int32_t xstage = 0;
-    auto fetch = [&](uint64_t b0) {
+    const uint64_t r0 = (uint64_t)blockIdx.y * a.slice_rows;
+    const uint64_t r1 = min(a.rows, r0 + a.slice_rows);
+    // this thread's pieces of a block: piece p = tid + 64 MF_WAVES j is row
+    // p / (4 KS), 16-byte chunk p % (4 KS).  Two register sets: a block is
+    // loaded two blocks ahead of its use, so its load has two blocks' worth
+    // of scoring to arrive before it is written to LDS.
+    constexpr uint32_t PIECES = (MF_BLK * 4 * KS + 64 * MF_WAVES - 1) / (64 * MF_WAVES);
+    v4i stage[2][PIECES];
+    int32_t wstage[2] = {MISSING, MISSING};
+    // register set R (a compile-time index: register arrays take no
+    // run-time index) <- block b0
+    auto fetch = [&](uint64_t b0, auto R) {
 #pragma unroll
-        for (int j = 0; j < KS; ++j) {
+        for (uint32_t j = 0; j < PIECES; ++j) {
             const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
             const uint64_t row = b0 + p / (4 * KS);
             const uint32_t off = 16 * (p % (4 * KS));
             v4i v = {0, 0, 0, 0};
-            if (row < r1 && off < dims) {
-                const uint4 u = *reinterpret_cast<const uint4 *>(base + row * dims + off);
-                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask), (int)(u.w ^ xmask)};
+            if (p < MF_BLK * 4 * KS && row < r1 && off < a.dims) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(a.base + row * a.row_step * a.dims +
+                                                                 off);
+                v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask),
+                        (int)(u.w ^ xmask)};
             }
-            stage[j] = v;
+            stage[R.value][j] = v;
         }
         if (threadIdx.x < MF_BLK) {
             const uint64_t row = b0 + threadIdx.x;
-            // a missing row fails every test
-            xstage = row < r1 ? (int32_t)nx[row] : (mips ? -(1 << 30) : (1 << 30));
+            const int32_t x = row < r1 ? (int32_t)a.nx[row * a.row_step] : 0;
+            wstage[R.value] = row < r1 ? (MIPS ? -x : x) : MISSING;
         }
     };
-    auto put = [&](int buf) {
+    // LDS buffer buf <- register set R
+    auto put = [&](int buf, auto R) {
 #pragma unroll
-        for (int j = 0; j < KS; ++j) {
+        for (uint32_t j = 0; j < PIECES; ++j) {
             const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
-            *reinterpret_cast<v4i *>(&rows_lds[buf][(p / (4 * KS)) * PITCH + 16 * (p % (4 * KS))]) =
-                stage[j];
+            if (p < MF_BLK * 4 * KS)
+                *reinterpret_cast<v4i *>(&rows_lds[buf][(p / (4 * KS)) * PITCH + 16 * (p % (4 * KS))]) =
+                    stage[R.value][j];
         }
         if (threadIdx.x < MF_BLK)
-            xs_lds[buf][threadIdx.x] = xstage;
+            w_lds[buf][threadIdx.x] = wstage[R.value];
     };
-    fetch(r0);
-    put(0);
-    __syncthreads();
-    int buf = 0;
-    for (uint64_t b0 = r0; b0 < r1; b0 += MF_BLK) {
-        const bool more = b0 + MF_BLK < r1;
-        if (more)
-            fetch(b0 + MF_BLK); // in flight while this block is scored
-        if (active) {
+    // every query tile against the block in LDS buffer buf (rows b0..)
+    auto score = [&](int buf, uint64_t b0) {
+        if (!active)
+            return;
 #pragma unroll
-            for (int blk = 0; blk < MF_BLK / 16; ++blk) {
-                const uint32_t rl = 16 * blk + row16;
-                const uint64_t row = b0 + rl;
-                v4i b[KS];
+        for (int blk = 0; blk < MF_BLK / 16; ++blk) {
+            v4i b[KS];
 #pragma unroll
-                for (int s = 0; s < KS; ++s)
-                    b[s] = *reinterpret_cast<const v4i *>(&rows_lds[buf][rl * PITCH + 64 * s + 16 * kq]);
-                const int32_t xs = xs_lds[buf][rl];
+            for (int s = 0; s < KS; ++s)
+                b[s] = *reinterpret_cast<const v4i *>(
+                    &rows_lds[buf][(16 * blk + col) * PITCH + 64 * s + 16 * kq]);
+            const v4i w4 = *reinterpret_cast<const v4i *>(&w_lds[buf][16 * blk + 4 * kq]);
+            // scan: the accumulator starts at -h (h = floor(w / 2), L2) or
+            // -w (inner product), shared by every query tile, so the matrix
+            // core returns c' = c - h or c - w.  e = w - 2c <= U implies
+            // c' >= -floor(U / 2) (a superset that adds only e = U + 1), and
+            // e = w - c <= U is c' >= -U: one compare on the largest of the
+            // four values per tile
+            v4i init = {0, 0, 0, 0};
+            if (!THRESH)
 #pragma unroll
-                for (int t = 0; t < MF_QW; ++t) {
-                    v4i c = {0, 0, 0, 0};
+                for (int r = 0; r < 4; ++r)
+                    init[r] = MIPS ? -w4[r] : -(w4[r] >> 1);
+            // chunk-major order: each accumulator's next product is issued
+            // MF_QW products after its last, so none waits on its own
+            // result; one wave-wide branch per sub-block
+            v4i c[MF_QW];
 #pragma unroll
-                    for (int s = 0; s < KS; ++s)
-                        c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t][s], b[s], c, 0, 0, 0);
+            for (int t = 0; t < MF_QW; ++t)
+                c[t] = init;
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int t = 0; t < MF_QW; ++t)
+                    c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b[s], qf[t][s], c[t], 0, 0, 0);
+            if (THRESH) {
+#pragma unroll
+                for (int t = 0; t < MF_QW; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const bool adm = mips ? c[r] + xs >= U[t][r] : xs - 2 * c[r] <= U[t][r];
-                        if (adm) {
-                            const uint32_t xn = (uint32_t)xs;
-                            const float d = mips ? -(float)(int32_t)((uint32_t)c[r] + qn[t][r] + xn)
-                                                 : (float)(int32_t)(qn[t][r] + xn - 2u * (uint32_t)c[r]);
-                            const uint64_t qr = q0 + 16 * t + 4 * kq + r;
-                            const uint32_t p = atomicAdd(&cnt[qr], 1u);
-                            if (p < MF_CAND)
-                                cand[qr * MF_CAND + p] = make_key(d, (uint32_t)row);
-                        }
+                        const int32_t e = MIPS ? w4[r] - c[t][r] : w4[r] - 2 * c[t][r];
+                        m2[t] = min(m2[t], max(m1[t], e));
+                        m1[t] = min(m1[t], e);
+                    }
+            } else {
+                bool hit = false;
+#pragma unroll
+                for (int t = 0; t < MF_QW; ++t)
+                    hit |= max(max(c[t][0], c[t][1]), max(c[t][2], c[t][3])) >= V[t];
+                if (ballot(hit)) {
+#pragma unroll
+                    for (int t = 0; t < MF_QW; ++t) {
+                        const uint64_t seg = ((q0 + 16 * t + col) * gridDim.y + blockIdx.y) * 4 + kq;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (c[t][r] >= V[t]) {
+                                // the exact partial distance, from c' and the row's w
+                                const int32_t e =
+                                    MIPS ? -c[t][r] : w4[r] - 2 * (c[t][r] + (w4[r] >> 1));
+                                const uint64_t row = b0 + 16 * blk + 4 * kq + r;
+                                if (kept[t] < a.seg_cap)
+                                    a.cand[seg * a.seg_cap + kept[t]] =
+                                        make_key((float)e, (uint32_t)row);
+                                ++kept[t];
+                            }
                     }
                 }
             }
         }
-        if (more)
-            put(buf ^ 1);
+    };
+    using R0 = std::integral_constant<int, 0>;
+    using R1 = std::integral_constant<int, 1>;
+    fetch(r0, R0());
+    fetch(r0 + MF_BLK, R1());
+    put(0, R0());
+    __syncthreads();
+    // blocks in pairs: the first of a pair is in LDS buffer 0, the second in
+    // buffer 1; register set 0 carries the block two ahead of buffer 0's
+    for (uint64_t b0 = r0; b0 < r1; b0 += 2 * MF_BLK) {
+        if (b0 + 2 * MF_BLK < r1)
+            fetch(b0 + 2 * MF_BLK, R0());
+        score(0, b0);
+        if (b0 + MF_BLK < r1)
+            put(1, R1());
         __syncthreads();
-        buf ^= 1;
+        if (b0 + MF_BLK >= r1)
+            break;
+        if (b0 + 3 * MF_BLK < r1)
+            fetch(b0 + 3 * MF_BLK, R1());
+        score(1, b0 + MF_BLK);
+        if (b0 + 2 * MF_BLK < r1)
+            put(0, R0());
+        __syncthreads();
+    }
+    if (!THRESH && active) {
+#pragma unroll
+        for (int t = 0; t < MF_QW; ++t) {
+            const uint64_t q = q0 + 16 * t + col;
+            if (q < a.nq)
+                a.segcnt[(q * gridDim.y + blockIdx.y) * 4 + kq] = kept[t];
+        }
+    }
+    if (THRESH && active) {
+        const uint32_t slices = gridDim.y;
+#pragma unroll
+        for (int t = 0; t < MF_QW; ++t) {
+            const uint64_t q = q0 + 16 * t + col;
+            if (q < a.nq) {
+                int32_t *o = a.tv + ((q * slices + blockIdx.y) * 4 + kq) * 2;
+                o[0] = m1[t];
+                o[1] = m2[t];
+            }
+        }
     }
 }
 
-// per query: sort the candidates, write k; flag overflowed queries
-__global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restrict__ cand,
-                                                       const uint32_t *__restrict__ cnt, uint32_t k,
-                                                       uint32_t *__restrict__ ids,
-                                                       float *__restrict__ dists,
-                                                       uint32_t *__restrict__ overflow)
+// per query (one wavefront): U = the J-th smallest of its nv values; the
+// redo list's length starts at zero
+__global__ __launch_bounds__(256) void mf_bound_kernel(const int32_t *__restrict__ tv, uint32_t nv,
+                                                     uint32_t J, uint64_t nq,
+                                                     int32_t *__restrict__ U,
+                                                     uint32_t *__restrict__ redo_cnt)
 {
-    __shared__ uint64_t s[MF_CAND];
-    const uint64_t q = blockIdx.x;
-    const uint32_t m = cnt[q];
-    if (m > MF_CAND) {
-        if (threadIdx.x == 0)
-            overflow[q] = 1;
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *redo_cnt = 0;
+    if (q >= nq)
         return;
+    constexpr int PER = 8 * MF_MAX_TSLICES / WAVE;
+    int32_t v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t i = lane + WAVE * j;
+        v[j] = i < nv ? tv[q * nv + i] : INT_MAX;
+    }
+    int32_t cur = INT_MAX;
+    for (uint32_t round = 0; round < J; ++round) {
+        int32_t m = v[0];
+#pragma unroll
+        for (int j = 1; j < PER; ++j)
+            m = min(m, v[j]);
+        int32_t w = m;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            w = min(w, __shfl_xor(w, o));
+        cur = w;
+        // the lowest lane holding the minimum drops one copy of it
+        const uint64_t holders = ballot(m == w);
+        if (lane == (uint32_t)__builtin_ctzll(holders)) {
+            bool done = false;
+#pragma unroll
+            for (int j = 0; j < PER; ++j)
+                if (!done && v[j] == w) {
+                    v[j] = INT_MAX;
+                    done = true;
+                }
+        }
+    }
+    if (lane == 0) {
+        // below the missing-row marker, so padding rows are never admitted
+        U[q] = min(cur, (1 << 30) - 1);
+    }
+}
+
+#define MF_SEL_MAX 4096 // rows one select workgroup sorts
+#define MF_SEL_SEGS 2048
+
+// per query: gather its nseg segments, sort the kept rows by (partial
+// distance, id), write the first k with their distances.  A query with an
+// overflowed segment, fewer than k kept rows or more than the workgroup
+// sorts goes on the redo list.
+template <bool MIPS>
+__global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restrict__ cand,
+                                                       const uint32_t *__restrict__ segcnt,
+                                                       uint32_t nseg, uint32_t seg_cap,
+                                                       const uint32_t *__restrict__ nqv,
+                                                       uint32_t k, uint32_t *__restrict__ ids,
+                                                       float *__restrict__ dists,
+                                                       uint32_t *__restrict__ redo_idx,
+                                                       uint32_t *__restrict__ redo_cnt)
+{
+    __shared__ uint64_t s[MF_SEL_MAX];
+    __shared__ uint32_t cnt[MF_SEL_SEGS], off[MF_SEL_SEGS];
+    __shared__ uint32_t bad, total;
+    const uint64_t q = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        bad = 0;
+        total = 0;
+    }
+    __syncthreads();
+    // segment offsets: counts, then an inclusive scan (one thread per
+    // stride of segments, then a serial pass over the per-thread sums)
+    const uint32_t per = (nseg + 255) / 256;
+    uint32_t mine = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t i = tid * per + j;
+        if (i < nseg) {
+            const uint32_t c = segcnt[q * nseg + i];
+            if (c > seg_cap)
+                bad = 1;
+            cnt[i] = c;
+            mine += c;
+        }
+    }
+    __shared__ uint32_t part[256];
+    part[tid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t t = 0; t < 256; ++t) {
+            const uint32_t v = part[t];
+            part[t] = acc;
+            acc += v;
+        }
+        total = acc;
+    }
+    __syncthreads();
+    const uint32_t m = total;
+    if (bad || m < k || m > MF_SEL_MAX) {
+        if (tid == 0)
+            redo_idx[atomicAdd(redo_cnt, 1u)] = (uint32_t)q;
+        return;
+    }
+    {
+        uint32_t acc = part[tid];
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t i = tid * per + j;
+            if (i < nseg) {
+                off[i] = acc;
+                acc += cnt[i];
+            }
+        }
+    }
+    __syncthreads();
+    // one wavefront per segment at a time, a lane per kept row
+    const uint32_t lane = tid % WAVE, wv = tid / WAVE;
+    for (uint32_t i = wv; i < nseg; i += 256 / WAVE) {
+        const uint32_t b = off[i], e = b + cnt[i];
+        for (uint32_t j = lane; b + j < e; j += WAVE)
+            s[b + j] = cand[(q * nseg + i) * seg_cap + j];
     }
     uint32_t n2 = 1;
     while (n2 < m)
         n2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
-        s[i] = i < m ? cand[q * MF_CAND + i] : ~0ull;
+    for (uint32_t i = m + tid; i < n2; i += 256)
+        s[i] = ~0ull;
     __syncthreads();
     for (uint32_t kk = 2; kk <= n2; kk <<= 1)
         for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+            for (uint32_t i = tid; i < n2; i += 256) {
                 const uint32_t p = i ^ j;
                 if (p > i) {
                     const uint64_t x = s[i], y = s[p];
@@ -274,11 +459,100 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
             }
             __syncthreads();
         }
-    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+    // partial distance back to the distance: + |q'|^2 (L2), - A(q) (inner product)
+    const int32_t qn = (int32_t)nqv[q];
+    for (uint32_t i = tid; i < k; i += 256) {
         ids[q * k + i] = key_id(s[i]);
-        if (dists)
-            dists[q * k + i] = key_dist(s[i]);
+        if (dists) {
+            const int32_t e = (int32_t)key_dist(s[i]);
+            dists[q * k + i] = (float)(MIPS ? e - qn : e + qn);
+        }
     }
+}
+
+// grow-only workspace per device, reused by every call on it (calls on one
+// device must not overlap in time)
+struct MfWorkspace {
+    uint8_t *p = nullptr;
+    size_t bytes = 0;
+};
+
+static int mf_workspace(size_t bytes, uint8_t **out)
+{
+    static std::mutex mu;
+    static MfWorkspace ws[64];
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return set_error(NBPG_EUNSUP, "device ordinal %d", dev);
+    std::lock_guard<std::mutex> g(mu);
+    MfWorkspace &w = ws[dev];
+    if (w.bytes < bytes) {
+        if (w.p) {
+            HIP_TRY(hipDeviceSynchronize()); // earlier calls' work may still use it
+            HIP_TRY(hipFree(w.p));
+            w.p = nullptr;
+            w.bytes = 0;
+        }
+        if (hipMalloc(&w.p, bytes) != hipSuccess)
+            return set_error(NBPG_ENOMEM, "MFMA ground-truth workspace of %zu bytes", bytes);
+        w.bytes = bytes;
+    }
+    *out = w.p;
+    return NBPG_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// the scan kernel for a row width and metric, and how many of its
+// workgroups one CU holds at once (registers and LDS; cached per kernel)
+template <bool THRESH> static const void *mf_kernel(uint32_t KS, bool mips)
+{
+    switch (KS * 2 + (mips ? 1 : 0)) {
+    case 2: return reinterpret_cast<const void *>(&mf_scan_kernel<1, false, THRESH>);
+    case 3: return reinterpret_cast<const void *>(&mf_scan_kernel<1, true, THRESH>);
+    case 4: return reinterpret_cast<const void *>(&mf_scan_kernel<2, false, THRESH>);
+    case 5: return reinterpret_cast<const void *>(&mf_scan_kernel<2, true, THRESH>);
+    case 6: return reinterpret_cast<const void *>(&mf_scan_kernel<3, false, THRESH>);
+    case 7: return reinterpret_cast<const void *>(&mf_scan_kernel<3, true, THRESH>);
+    case 8: return reinterpret_cast<const void *>(&mf_scan_kernel<4, false, THRESH>);
+    default: return reinterpret_cast<const void *>(&mf_scan_kernel<4, true, THRESH>);
+    }
+}
+
+static uint32_t mf_per_cu(const void *fn)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<const void *, uint32_t>> cache;
+    std::lock_guard<std::mutex> g(mu);
+    for (const auto &e : cache)
+        if (e.first == fn)
+            return e.second;
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64 * MF_WAVES, 0) != hipSuccess || b < 1)
+        b = 1;
+    cache.emplace_back(fn, (uint32_t)b);
+    return (uint32_t)b;
+}
+
+// slices of `rows` for `tiles` query tiles: as many workgroups as the device
+// holds at once (per_cu per CU, so none waits for a second round), at
+// least `min_blocks` staged blocks per slice, at most max_slices
+static uint64_t mf_slice_rows(uint64_t rows, uint64_t tiles, uint32_t per_cu, uint64_t min_blocks,
+                              uint64_t max_slices)
+{
+    uint64_t slices = std::max<uint64_t>(1, (uint64_t)per_cu * g_dev.cu / tiles);
+    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, rows / (MF_BLK * min_blocks)));
+    slices = std::min<uint64_t>(slices, max_slices);
+    return ((rows + slices - 1) / slices + MF_BLK - 1) / MF_BLK * MF_BLK;
+}
+
+static int mf_launch(const void *fn, dim3 grid, hipStream_t s, MfArgs &a)
+{
+    void *args[] = {&a};
+    if (hipLaunchKernel(fn, grid, dim3(64 * MF_WAVES), args, 0, s) != hipSuccess)
+        return set_error(NBPG_EHIP, "MFMA ground-truth scan launch");
+    return NBPG_OK;
 }
 
 int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queries, uint64_t nq,
@@ -292,112 +566,118 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         return set_error(NBPG_EUNSUP, "MFMA ground truth needs 16-byte rows (dims %u)", dims);
     const int is_u8 = dtype == NBPG_U8;
     const uint32_t KS = (dims + 63) / 64;
-    // 1. thresholds from every step-th row: about k * n / S rows then pass
-    //    per query, kept to a quarter of the candidate slots
-    const uint64_t want = std::max<uint64_t>(
-        std::max<uint64_t>(131072, 64ull * k), (4ull * k * n + MF_CAND - 1) / MF_CAND);
-    const uint64_t step = std::max<uint64_t>(1, n / want);
-    const uint64_t S = (n + step - 1) / step; // >= min(n, want) >= k
-    uint32_t *tid = nullptr, *cnt = nullptr, *nx = nullptr, *nqv = nullptr, *ovf = nullptr;
-    float *td = nullptr;
-    uint64_t *thr = nullptr, *cand = nullptr;
-    auto cleanup = [&]() {
-        hipFreeAsync(tid, s);
-        hipFreeAsync(td, s);
-        hipFreeAsync(cnt, s);
-        hipFreeAsync(nx, s);
-        hipFreeAsync(nqv, s);
-        hipFreeAsync(ovf, s);
-        hipFreeAsync(thr, s);
-        hipFreeAsync(cand, s);
+    const bool im = mips != 0;
+    // sample stride: the bound's sample rank is about J, so about J * step
+    // (4k) base rows are kept
+    const uint64_t step = std::max<uint64_t>(1, std::min<uint64_t>(4ull * k / MF_J, n / 64));
+    const uint64_t S = (n + step - 1) / step;
+    // equal chunks of at most MF_QCHUNK queries
+    const uint64_t nchunks = (nq + MF_QCHUNK - 1) / MF_QCHUNK;
+    const uint64_t chunk = (nq + nchunks - 1) / nchunks;
+    const uint64_t last = nq - (nq - 1) / chunk * chunk;
+    const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
+    const void *thr_fn = mf_kernel<true>(KS, im), *scan_fn = mf_kernel<false>(KS, im);
+    // scan geometry of a chunk of m queries: slices of the base, and the
+    // segment size for about 4k kept rows per query spread over 4 segments
+    // per slice (room for three times the mean, at least 16)
+    struct Geo {
+        uint64_t tiles, slice_rows;
+        uint32_t slices, seg_cap;
     };
-    bool ok = hipMallocAsync((void **)&tid, nq * k * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&td, nq * k * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&cnt, nq * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&nx, n * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&nqv, nq * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&ovf, nq * 4, s) == hipSuccess &&
-              hipMallocAsync((void **)&thr, nq * 8, s) == hipSuccess &&
-              hipMallocAsync((void **)&cand, nq * MF_CAND * 8, s) == hipSuccess;
-    if (!ok) {
-        cleanup();
-        return set_error(NBPG_ENOMEM, "MFMA ground-truth workspace");
-    }
-    int rc = compgt_tiles(dtype, base, S, step * dims, queries, nq, dims, k, true, mips, tid, td, s);
-    if (rc != NBPG_OK) {
-        cleanup();
-        return rc;
-    }
-    hipLaunchKernelGGL(mf_thresholds_kernel, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s,
-                       td, nq, k, thr, cnt, ovf);
+    auto geo = [&](uint64_t m) {
+        Geo g;
+        g.tiles = (m + qpb - 1) / qpb;
+        g.slice_rows = mf_slice_rows(n, g.tiles, mf_per_cu(scan_fn), 4, MF_SEL_SEGS / 4);
+        g.slices = (uint32_t)((n + g.slice_rows - 1) / g.slice_rows);
+        const uint64_t want = std::max<uint64_t>(16, 3ull * k / g.slices + 16);
+        g.seg_cap = 16;
+        while (g.seg_cap < want)
+            g.seg_cap <<= 1;
+        return g;
+    };
+    const Geo gc = geo(chunk), gl = geo(last);
+    const uint64_t seg_keys = std::max<uint64_t>(chunk * gc.slices * 4 * gc.seg_cap,
+                                                 last * gl.slices * 4 * gl.seg_cap);
+    const uint64_t segs = std::max<uint64_t>(chunk * gc.slices, last * gl.slices) * 4;
+    // workspace: nx | qn | U | redo count + list | tv | segment counts |
+    // segments | the redo's partial buffer (for a full and for the last chunk)
+    size_t off[9];
+    off[0] = 0;
+    off[1] = off[0] + align256(n * 4);
+    off[2] = off[1] + align256(chunk * 4);
+    off[3] = off[2] + align256(chunk * 4);
+    off[4] = off[3] + align256((chunk + 1) * 4);
+    off[5] = off[4] + align256(chunk * 8 * MF_MAX_TSLICES * 4);
+    off[6] = off[5] + align256(segs * 4);
+    off[7] = off[6] + align256(seg_keys * 8);
+    off[8] = off[7] + align256(std::max(compgt_plain_partial_bytes(chunk, n, k, true),
+                                        compgt_plain_partial_bytes(last, n, k, true)));
+    uint8_t *ws = nullptr;
+    NBPG_TRY(mf_workspace(off[8], &ws));
+    uint32_t *nx = reinterpret_cast<uint32_t *>(ws + off[0]);
+    uint32_t *nqv = reinterpret_cast<uint32_t *>(ws + off[1]);
+    int32_t *U = reinterpret_cast<int32_t *>(ws + off[2]);
+    uint32_t *redo_cnt = reinterpret_cast<uint32_t *>(ws + off[3]);
+    uint32_t *redo_idx = redo_cnt + 1;
+    int32_t *tv = reinterpret_cast<int32_t *>(ws + off[4]);
+    uint32_t *segcnt = reinterpret_cast<uint32_t *>(ws + off[5]);
+    uint64_t *cand = reinterpret_cast<uint64_t *>(ws + off[6]);
+    uint64_t *partial = reinterpret_cast<uint64_t *>(ws + off[7]);
+
     hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((n + 31) / 32)), dim3(256), 0, s, base, n,
                        dims, is_u8, mips, nx);
-    hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((nq + 31) / 32)), dim3(256), 0, s,
-                       queries, nq, dims, is_u8, mips, nqv);
-    // 2. MFMA scan: query tiles x base slices
-    const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
-    const uint64_t tiles = (nq + qpb - 1) / qpb;
-    uint64_t slices = std::max<uint64_t>(1, (uint64_t)g_dev.cu * 8 / tiles);
-    slices = std::min<uint64_t>(slices, std::max<uint64_t>(1, n / 4096));
-    const uint64_t slice_rows = ((n + slices - 1) / slices + MF_BLK - 1) / MF_BLK * MF_BLK;
-    slices = (n + slice_rows - 1) / slice_rows;
-    dim3 grid((uint32_t)tiles, (uint32_t)slices), block(64 * MF_WAVES);
-    switch (KS) {
-    case 1: hipLaunchKernelGGL((mf_scan_kernel<1>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
-    case 2: hipLaunchKernelGGL((mf_scan_kernel<2>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
-    case 3: hipLaunchKernelGGL((mf_scan_kernel<3>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
-    default: hipLaunchKernelGGL((mf_scan_kernel<4>), grid, block, 0, s, base, n, queries, nq, dims, is_u8, nx, nqv, thr, slice_rows, cnt, cand, mips); break;
+    for (uint64_t c0 = 0; c0 < nq; c0 += chunk) {
+        const uint64_t m = std::min<uint64_t>(chunk, nq - c0);
+        const Geo g = m == chunk ? gc : gl;
+        const uint8_t *qc = queries + c0 * dims;
+        uint32_t *ids = out_ids + c0 * k;
+        float *dists = out_dists ? out_dists + c0 * k : nullptr;
+        hipLaunchKernelGGL(mf_norms_kernel, dim3((uint32_t)((m + 31) / 32)), dim3(256), 0, s, qc,
+                           m, dims, is_u8, mips, nqv);
+        MfArgs a;
+        memset(&a, 0, sizeof(a));
+        a.base = base;
+        a.queries = qc;
+        a.nq = m;
+        a.dims = dims;
+        a.is_u8 = is_u8;
+        a.nx = nx;
+        a.nqv = nqv;
+        a.U = U;
+        a.cand = cand;
+        a.segcnt = segcnt;
+        a.seg_cap = g.seg_cap;
+        a.tv = tv;
+        // 1. bound from the sample
+        a.rows = S;
+        a.row_step = step;
+        a.slice_rows = mf_slice_rows(S, g.tiles, mf_per_cu(thr_fn), 1, MF_MAX_TSLICES);
+        const uint32_t tslices = (uint32_t)((S + a.slice_rows - 1) / a.slice_rows);
+        NBPG_TRY(mf_launch(thr_fn, dim3((uint32_t)g.tiles, tslices), s, a));
+        const uint32_t nv = 8 * tslices;
+        hipLaunchKernelGGL(mf_bound_kernel, dim3((uint32_t)((m + 3) / 4)), dim3(256), 0, s, tv, nv,
+                           std::min<uint32_t>(MF_J, nv), m, U, redo_cnt);
+        // 2. scan
+        a.rows = n;
+        a.row_step = 1;
+        a.slice_rows = g.slice_rows;
+        NBPG_TRY(mf_launch(scan_fn, dim3((uint32_t)g.tiles, g.slices), s, a));
+        // 3. select
+        if (im)
+            hipLaunchKernelGGL(mf_select_kernel<true>, dim3((uint32_t)m), dim3(256), 0, s, cand,
+                               segcnt, 4 * g.slices, g.seg_cap, nqv, k, ids, dists, redo_idx,
+                               redo_cnt);
+        else
+            hipLaunchKernelGGL(mf_select_kernel<false>, dim3((uint32_t)m), dim3(256), 0, s, cand,
+                               segcnt, 4 * g.slices, g.seg_cap, nqv, k, ids, dists, redo_idx,
+                               redo_cnt);
+        if (hipGetLastError() != hipSuccess)
+            return set_error(NBPG_EHIP, "MFMA ground-truth launch");
+        // 4. the listed queries by the plain scan
+        NBPG_TRY(compgt_plain_core(dtype, base, n, dims, qc, m, dims, k, mips, ids, dists, s,
+                                   partial, redo_idx, redo_cnt));
     }
-    if (hipGetLastError() != hipSuccess) {
-        cleanup();
-        return set_error(NBPG_EHIP, "MFMA scan launch");
-    }
-    // 3. select, then redo overflowed queries with the plain scan
-    hipLaunchKernelGGL(mf_select_kernel, dim3((uint32_t)nq), dim3(256), 0, s, cand, cnt, k, out_ids,
-                       out_dists, ovf);
-    std::vector<uint32_t> h(nq);
-    if (hipMemcpyAsync(h.data(), ovf, nq * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        cleanup();
-        return set_error(NBPG_EHIP, "overflow readback");
-    }
-    std::vector<uint32_t> redo;
-    for (uint64_t q = 0; q < nq; ++q)
-        if (h[q])
-            redo.push_back((uint32_t)q);
-    if (!redo.empty()) {
-        const uint32_t m = (uint32_t)redo.size();
-        uint32_t *idx = nullptr, *cids = nullptr;
-        float *cd = nullptr;
-        uint8_t *rows = nullptr;
-        ok = hipMallocAsync((void **)&idx, m * 4ull, s) == hipSuccess &&
-             hipMallocAsync((void **)&cids, (uint64_t)m * k * 4, s) == hipSuccess &&
-             hipMallocAsync((void **)&cd, (uint64_t)m * k * 4, s) == hipSuccess &&
-             hipMallocAsync((void **)&rows, (uint64_t)m * dims, s) == hipSuccess &&
-             hipMemcpyAsync(idx, redo.data(), m * 4ull, hipMemcpyHostToDevice, s) == hipSuccess;
-        if (ok) {
-            const uint64_t tb = (uint64_t)m * dims, tk = (uint64_t)m * k;
-            hipLaunchKernelGGL(mf_gather_kernel, dim3((uint32_t)((tb + 255) / 256)), dim3(256), 0,
-                               s, queries, idx, m, dims, rows);
-            rc = compgt_plain_launch(dtype, base, n, dims, rows, m, dims, k, mips, cids, cd, s);
-            if (rc == NBPG_OK)
-                hipLaunchKernelGGL(mf_scatter_kernel, dim3((uint32_t)((tk + 255) / 256)),
-                                   dim3(256), 0, s, cids, cd, idx, m, k, out_ids, out_dists);
-            // the host vector redo must outlive the copy
-            if (hipStreamSynchronize(s) != hipSuccess && rc == NBPG_OK)
-                rc = set_error(NBPG_EHIP, "overflow redo");
-        } else {
-            rc = set_error(NBPG_ENOMEM, "overflow redo workspace");
-        }
-        hipFreeAsync(idx, s);
-        hipFreeAsync(cids, s);
-        hipFreeAsync(cd, s);
-        hipFreeAsync(rows, s);
-        if (!ok)
-            hipStreamSynchronize(s);
-    }
-    cleanup();
-    return rc;
+    return NBPG_OK;
 }
 
 } // namespace nbpg

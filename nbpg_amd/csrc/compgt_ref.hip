@@ -32,7 +32,8 @@ namespace nbpg {
 
 // compgt.hip: sorts each query's slices x k keys and writes the first k
 int compgt_merge_launch(const uint64_t *partial, uint64_t nq, uint32_t slices, uint32_t k,
-                        uint32_t *ids, float *dists, hipStream_t s);
+                        uint32_t *ids, float *dists, hipStream_t s,
+                        const uint32_t *qidx = nullptr, const uint32_t *qcount = nullptr);
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 

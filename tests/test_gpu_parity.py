@@ -309,6 +309,33 @@ def test_compgt_mfma_shapes_and_overflow(gpu):
                  N.GT_MFMA)
 
 
+def test_compgt_mfma_bound_too_tight_and_chunks(gpu):
+    """Rows near a query placed only where the threshold sample looks (every
+    16th row for k = 100): the sample's bound admits fewer than k rows, the
+    query goes to the plain redo on the device, and the result is still the
+    exact top-k.  More queries than one MFMA chunk (16384) in one call; the
+    second call with fewer queries reuses the workspace."""
+    N = gpu
+    rng = np.random.default_rng(19)
+    n, dims = 50000, 64
+    base = rng.integers(0, 256, (n, dims), dtype=np.uint8)
+    q = rng.integers(0, 256, (40, dims), dtype=np.uint8)
+    near = np.clip(q[0].astype(np.int32) + rng.integers(-2, 3, (30, dims)), 0, 255)
+    base[16 * np.arange(30)] = near.astype(np.uint8)
+    for nqq in (40, 3):
+        ids, d = N.compgt(N.to_device(base), N.to_device(q[:nqq]), 100, N.U8, N.GT_MFMA)
+        oi, od = O.compgt(base, q[:nqq], 100, O.U8, tie_mode=1)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), nqq
+        assert (d.cpu().numpy() == od).all(), nqq
+    base = rng.integers(0, 256, (3000, 32), dtype=np.uint8)
+    q = rng.integers(0, 256, (20000, 32), dtype=np.uint8)
+    for nqq, k in ((20000, 10), (16385, 50), (100, 10)):
+        ids, d = N.compgt(N.to_device(base), N.to_device(q[:nqq]), k, N.U8, N.GT_MFMA)
+        oi, od = O.compgt(base, q[:nqq], k, O.U8, tie_mode=1)
+        assert (ids.cpu().numpy().view(np.uint32) == oi).all(), (nqq, k)
+        assert (d.cpu().numpy() == od).all(), (nqq, k)
+
+
 @pytest.mark.parametrize("engine", [0, 1])
 def test_compgt_reference_rule(gpu, golden, engine):
     """NBPG_GT_REFTIES replays CompGT.cpp:63-97: the compiled reference's own
