@@ -40,9 +40,17 @@ def load_mat(path):
     return W, bias
 
 
-def fit(x_dev, d_out, sample=100000, margin=1.0):
+def fit(x_dev, d_out, sample=100000, margin=1.0, clip=1.0):
     """Returns dict(W f32 d_out x d_in, bias f32, eigenvalues f64 (all,
-    descending), scale, mean f64)."""
+    descending), scale, mean f64).
+
+    The scale maps the `clip` quantile of |projection| over a sample (all
+    components together) to 1: clip = 1 takes the largest value, so every
+    projection of the sample lies in [-1, 1]; a smaller clip lets the
+    quantizer saturate the largest values (compress clamps them,
+    matrix.h:99-127) in exchange for finer steps everywhere else, which
+    4-bit codes need.  One global scale either way, so the code space stays
+    a scaled copy of the PCA space."""
     import torch
     mean, cov = N.pca_stats(x_dev)
     torch.cuda.synchronize()
@@ -57,7 +65,12 @@ def fit(x_dev, d_out, sample=100000, margin=1.0):
     mean_h = mean.cpu().numpy()
     xs = x_dev[:sample].to(torch.float64)
     proj = (xs - torch.from_numpy(mean_h).to(xs.device)) @ torch.from_numpy(V.T.copy()).to(xs.device)
-    scale = float(proj.abs().max().item()) * margin
+    a = proj.abs().flatten()
+    if clip >= 1.0:
+        scale = float(a.max().item()) * margin
+    else:
+        k = max(1, int(round(clip * a.numel())))
+        scale = float(a.kthvalue(k).values.item()) * margin
     scale = scale if scale > 0 else 1.0
     W = V / scale
     bias = -(W @ mean_h)

@@ -170,3 +170,24 @@ def test_fit_recovers_subspace(gpu):
     _, y = N.pca_project(torch.from_numpy(x).cuda(), torch.from_numpy(f["W"]).cuda(),
                          torch.from_numpy(f["bias"]).cuda(), code=N.CODE_F32, mode=N.PCA_PLAIN)
     assert np.abs(y.cpu().numpy()).max() <= 1.0 + 1e-5
+
+
+def test_fit_clip_quantile(gpu):
+    """fit(clip=q) maps the q-quantile of the sample's |projection| to 1: a
+    fraction 1 - q of the sample's projected values lands outside [-1, 1]
+    (the quantizer saturates them), the directions are the clip = 1 fit's,
+    and the scale only shrinks."""
+    N = gpu
+    import torch
+    from nbpg_amd import pca as P
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal((30000, 48)) * np.linspace(3, 0.2, 48)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    f1 = P.fit(xd, 16, sample=30000)
+    for q in (0.999, 0.99, 0.95):
+        f = P.fit(xd, 16, sample=30000, clip=q)
+        assert f["scale"] < f1["scale"]
+        assert np.allclose(f["W"] * f["scale"], f1["W"] * f1["scale"], atol=1e-6)
+        y = (x.astype(np.float64) @ f["W"].astype(np.float64).T + f["bias"])
+        out = float((np.abs(y) > 1.0).mean())
+        assert abs(out - (1 - q)) < 0.002 + 0.1 * (1 - q), (q, out)
