@@ -45,9 +45,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 #define MF_WAVES 8 // wavefronts per workgroup: 512 queries share each staged block
 #endif
 #ifndef MF_BLK
-#define MF_BLK 64 // base rows per block staged through LDS
+#define MF_BLK 128 // base rows per block staged through LDS
 #endif
-#define MF_J 24       // bound = the J-th smallest of the per-lane top-2 sample values
+#define MF_J 16 // bound = the J-th smallest of the per-lane top-2 sample values
 #define MF_QCHUNK 16384
 #define MF_MAX_TSLICES 64
 
@@ -379,65 +379,67 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
                                                        uint32_t *__restrict__ redo_cnt)
 {
     __shared__ uint64_t s[MF_SEL_MAX];
-    __shared__ uint32_t cnt[MF_SEL_SEGS], off[MF_SEL_SEGS];
-    __shared__ uint32_t bad, total;
+    __shared__ uint32_t wsum[256 / WAVE];
+    __shared__ uint32_t bad;
     const uint64_t q = blockIdx.x;
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) {
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+    if (tid == 0)
         bad = 0;
-        total = 0;
-    }
-    __syncthreads();
-    // segment offsets: counts, then an inclusive scan (one thread per
-    // stride of segments, then a serial pass over the per-thread sums)
+    // thread tid owns segments [tid * per, tid * per + per): their counts,
+    // then a block-wide exclusive scan of the per-thread sums
     const uint32_t per = (nseg + 255) / 256;
+    uint32_t c[MF_SEL_SEGS / 256];
     uint32_t mine = 0;
-    for (uint32_t j = 0; j < per; ++j) {
+#pragma unroll
+    for (uint32_t j = 0; j < MF_SEL_SEGS / 256; ++j) {
         const uint32_t i = tid * per + j;
-        if (i < nseg) {
-            const uint32_t c = segcnt[q * nseg + i];
-            if (c > seg_cap)
-                bad = 1;
-            cnt[i] = c;
-            mine += c;
-        }
-    }
-    __shared__ uint32_t part[256];
-    part[tid] = mine;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t t = 0; t < 256; ++t) {
-            const uint32_t v = part[t];
-            part[t] = acc;
-            acc += v;
-        }
-        total = acc;
+        c[j] = j < per && i < nseg ? segcnt[q * nseg + i] : 0u;
+        mine += c[j];
     }
     __syncthreads();
-    const uint32_t m = total;
+    bool over = false;
+#pragma unroll
+    for (uint32_t j = 0; j < MF_SEL_SEGS / 256; ++j)
+        over |= c[j] > seg_cap;
+    if (over)
+        bad = 1;
+    uint32_t x = mine;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o)
+            x += y;
+    }
+    if (lane == WAVE - 1)
+        wsum[wv] = x;
+    __syncthreads();
+    uint32_t start = x - mine, m = 0;
+    for (uint32_t w = 0; w < 256 / WAVE; ++w) {
+        start += w < wv ? wsum[w] : 0u;
+        m += wsum[w];
+    }
     if (bad || m < k || m > MF_SEL_MAX) {
         if (tid == 0)
             redo_idx[atomicAdd(redo_cnt, 1u)] = (uint32_t)q;
         return;
     }
-    {
-        uint32_t acc = part[tid];
-        for (uint32_t j = 0; j < per; ++j) {
-            const uint32_t i = tid * per + j;
-            if (i < nseg) {
-                off[i] = acc;
-                acc += cnt[i];
+    // each thread copies its own segments' kept rows, four loads in flight
+#pragma unroll
+    for (uint32_t j = 0; j < MF_SEL_SEGS / 256; ++j) {
+        const uint32_t i = tid * per + j;
+        if (j < per && i < nseg)
+            for (uint32_t e0 = 0; e0 < c[j]; e0 += 4) {
+                uint64_t v[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+                    if (e0 + u < c[j])
+                        v[u] = cand[(q * nseg + i) * seg_cap + e0 + u];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+                    if (e0 + u < c[j])
+                        s[start + e0 + u] = v[u];
             }
-        }
-    }
-    __syncthreads();
-    // one wavefront per segment at a time, a lane per kept row
-    const uint32_t lane = tid % WAVE, wv = tid / WAVE;
-    for (uint32_t i = wv; i < nseg; i += 256 / WAVE) {
-        const uint32_t b = off[i], e = b + cnt[i];
-        for (uint32_t j = lane; b + j < e; j += WAVE)
-            s[b + j] = cand[(q * nseg + i) * seg_cap + j];
+        start += c[j];
     }
     uint32_t n2 = 1;
     while (n2 < m)
