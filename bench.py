@@ -46,7 +46,8 @@ def main():
     ap.add_argument("--L", type=int, default=0, help="fixed beam width (0 = sweep)")
     ap.add_argument("--fast", default="auto",
                     help="waves_per_group,expand_width,filter_log2,groups,row_split or 'auto'")
-    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a hipGraph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches instead of the library's captured search graph")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-entry0", action="store_true",
                     help="skip timing the FAST path from the single entry node 0 (profiling runs)")
@@ -195,19 +196,13 @@ def main():
     L_e0, rec_e0, _ = pick(run_fast_entry0, "fast, entry node 0") if cmp_e0 \
         else (L_fast, rec_fast, None)
 
-    def timed(runner, L, steps, warmup, graph):
+    def timed(runner, L, steps, warmup, graph_of=None):
         out = runner(L)
         step = lambda: runner(L, out)  # noqa: E731
-        if graph:
-            # one step captured in a hipGraph, replayed per step
-            s = torch.cuda.Stream()
-            s.wait_stream(torch.cuda.current_stream())
-            with torch.cuda.stream(s):
-                runner(L, out)
-            torch.cuda.current_stream().wait_stream(s)
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                runner(L, out)
+        if graph_of is not None:
+            # one step captured in a HIP graph by the library itself
+            # (nbpg_search_graph_*), replayed per step on torch's stream
+            g = graph_of(L)
             step = g.replay
         for _ in range(warmup):
             step()
@@ -225,7 +220,14 @@ def main():
         per = np.array([a.elapsed_time(b) for a, b in ev])
         return wall, per
 
-    wall, per = timed(run_fast, L_fast, args.steps, args.warmup, not args.no_graph)
+    def graph_fast(L):
+        return N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, start_points=seeds)
+
+    def graph_fast_entry0(L):
+        return N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast)
+
+    wall, per = timed(run_fast, L_fast, args.steps, args.warmup,
+                      None if args.no_graph else graph_fast)
     nq = q.shape[0]
     qps = nq * args.steps / wall
     log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
@@ -259,10 +261,11 @@ def main():
         % (nq / (e2e.mean() * 1e-3), np.percentile(e2e, 50), np.percentile(e2e, 99)))
     e0_qps = qps
     if cmp_e0:
-        w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2, not args.no_graph)
+        w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2,
+                      None if args.no_graph else graph_fast_entry0)
         e0_qps = nq * max(5, args.steps // 2) / w0
         log("fast with the single entry node 0, L=%d: %.0f QPS" % (L_e0, e0_qps))
-    pwall, pper = timed(run_plain, L_plain, args.plain_steps, 1, False)
+    pwall, pper = timed(run_plain, L_plain, args.plain_steps, 1)
     plain_qps = nq * args.plain_steps / pwall
     log("plain L=%d: %.0f QPS" % (L_plain, plain_qps))
 

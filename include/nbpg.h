@@ -197,6 +197,24 @@ int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq
                        const nbpg_query_params *qp, uint32_t rerank_factor, int mode,
                        const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
                        uint32_t *d_out_stats, void *stream);
+/* ---- one nbpg_search captured as a HIP graph and replayed: the batched
+ * driver qsearchAll (beamSearch.h:257-303) as a single graph launch per
+ * batch.  create makes one eager call (argument checks, workspace growth),
+ * then captures the same call; launch replays it on `stream`, reading
+ * whatever d_queries holds at that time and writing the same outputs.  The
+ * index, buffers and start points are fixed at creation; the FAST kernel's
+ * work counter returns itself to zero, so the graph holds no memset.
+ * Launches of one graph, and of any search on the same index, must not
+ * overlap in time. */
+typedef struct nbpg_search_graph nbpg_search_graph;
+int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *idx, const void *d_queries,
+                             uint64_t nq, uint64_t query_stride, const uint32_t *h_start_points,
+                             uint32_t sp_num, const nbpg_query_params *qp, int mode,
+                             const nbpg_fast_config *fast, uint32_t *d_out_ids,
+                             float *d_out_dists, uint32_t *d_out_stats);
+int nbpg_search_graph_launch(nbpg_search_graph *g, void *stream);
+int nbpg_search_graph_destroy(nbpg_search_graph *g);
+
 /* nbpg_search_rerank without any allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -86,6 +86,11 @@ _SIGS = {
                                  C.POINTER(QueryParams), U32, I32, VP, VP, VP, VP, VP]),
     "nbpg_search_rerank_noalloc": (I32, [VP, VP, U64, U64, I32, VP, U32, VP, U64, VP, U32,
                                          C.POINTER(QueryParams), U32, I32, VP, VP, VP, VP, VP]),
+    "nbpg_search_graph_create": (I32, [C.POINTER(VP), VP, VP, U64, U64, VP, U32,
+                                       C.POINTER(QueryParams), I32, C.POINTER(FastConfig),
+                                       VP, VP, VP]),
+    "nbpg_search_graph_launch": (I32, [VP, VP]),
+    "nbpg_search_graph_destroy": (I32, [VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
                           C.POINTER(FastConfig), VP, VP, VP, VP]),
     "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, U32, C.POINTER(FastConfig),
@@ -260,6 +265,48 @@ def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_lim
                              C.byref(qp), mode, C.byref(fc) if fc is not None else None,
                              _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search")
     return ids, d, st
+
+
+class SearchGraph:
+    """One search captured as a HIP graph (nbpg_search_graph_*): replay()
+    searches whatever q_dev holds into the same outputs (ids, dists, stats)."""
+
+    def __init__(self, index, q_dev, L, k, mode=SEARCH_FAST, cut=1.35, limit=None,
+                 degree_limit=None, entry=0, start_points=None, fast=None, edges=False):
+        torch = _torch()
+        nq = q_dev.shape[0]
+        qstride = q_dev.shape[1] * q_dev.element_size()
+        qp = QueryParams(k, cut, L, index.n if limit is None else limit,
+                         index.R if degree_limit is None else degree_limit)
+        sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
+        if edges:
+            mode |= SEARCH_EDGES
+        self.ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        self.dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        self.stats = torch.empty((nq, 3 if edges else 2), dtype=torch.int32, device="cuda")
+        fc = FastConfig(*fast) if fast is not None else None
+        h = VP()
+        _check(lib().nbpg_search_graph_create(C.byref(h), index.h, _ptr(q_dev), nq, qstride,
+                                              sp.ctypes.data, len(sp), C.byref(qp), mode,
+                                              C.byref(fc) if fc is not None else None,
+                                              _ptr(self.ids), _ptr(self.dists),
+                                              _ptr(self.stats)), "nbpg_search_graph_create")
+        self.h, self.index, self.q_dev = h, index, q_dev
+
+    def replay(self):
+        _check(lib().nbpg_search_graph_launch(self.h, _stream()), "nbpg_search_graph_launch")
+        return self.ids, self.dists, self.stats
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nbpg_search_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100,

@@ -694,3 +694,79 @@ int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// one search captured as a HIP graph, replayed per batch
+// ---------------------------------------------------------------------------
+
+struct nbpg_search_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *ix, const void *d_queries,
+                             uint64_t nq, uint64_t query_stride, const uint32_t *h_start_points,
+                             uint32_t sp_num, const nbpg_query_params *qp, int mode,
+                             const nbpg_fast_config *fast, uint32_t *d_out_ids,
+                             float *d_out_dists, uint32_t *d_out_stats)
+{
+    if (!out)
+        return set_error(NBPG_EINVAL, "null output handle");
+    *out = nullptr;
+    NBPG_TRY(ensure_init());
+    if (nq == 0)
+        return set_error(NBPG_EINVAL, "empty batch");
+    // one eager call first: it checks the arguments and grows the index's
+    // workspace (reference mode beyond LDS), so the capture allocates nothing
+    NBPG_TRY(nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
+                         d_out_ids, d_out_dists, d_out_stats, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    nbpg_search_graph *g = new nbpg_search_graph();
+    int rc = NBPG_OK;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = set_error(NBPG_EHIP, "graph capture start");
+    } else {
+        rc = nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
+                         d_out_ids, d_out_dists, d_out_stats, s);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc == NBPG_OK && e != hipSuccess)
+            rc = set_error(NBPG_EHIP, "graph capture: %s", hipGetErrorString(e));
+        g->graph = graph;
+        if (rc == NBPG_OK && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = set_error(NBPG_EHIP, "graph instantiation");
+    }
+    hipStreamDestroy(s);
+    if (rc != NBPG_OK) {
+        nbpg_search_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return NBPG_OK;
+}
+
+int nbpg_search_graph_launch(nbpg_search_graph *g, void *stream)
+{
+    if (!g || !g->exec)
+        return set_error(NBPG_EINVAL, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, as_stream(stream)));
+    return NBPG_OK;
+}
+
+int nbpg_search_graph_destroy(nbpg_search_graph *g)
+{
+    if (!g)
+        return NBPG_OK;
+    if (g->exec)
+        hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        hipGraphDestroy(g->graph);
+    delete g;
+    return NBPG_OK;
+}
+
+} // extern "C"

@@ -615,3 +615,32 @@ def test_fast_edge_counts(gpu, cfg):
         with pytest.raises(N.NbpgError):
             N.search(ix, qd, 32, 10, mode=N.SEARCH_REF, edges=True)
         ix.close()
+
+
+@pytest.mark.parametrize("mode,L", [("fast", 32), ("ref", 32), ("ref", 256), ("fast", 300)])
+def test_search_graph_replay_equals_eager(gpu, small, mode, L):
+    """nbpg_search_graph_*: the captured search replayed several times, with
+    eager searches and synchronisations between the replays and new query
+    rows written into the captured buffer, equals the eager search of the
+    same rows every time (ids, distances, counters)."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    m = N.SEARCH_FAST if mode == "fast" else N.SEARCH_REF
+    seeds = N.pick_seeds(N.to_device(base), 16) if mode == "fast" and L <= 256 else None
+    qa = dev_rows(N, q[:100], ix.row_stride)
+    qb = dev_rows(N, q[100:200], ix.row_stride)
+    buf = qa.clone()
+    g = N.SearchGraph(ix, buf, L, 10, mode=m, start_points=seeds)
+    for rep in range(4):
+        src = qa if rep % 2 == 0 else qb
+        buf.copy_(src)
+        got = [t.clone() for t in g.replay()]
+        torch.cuda.synchronize()
+        want = N.search(ix, src, L, 10, mode=m, start_points=seeds)
+        torch.cuda.synchronize()
+        for a, b in zip(got, want):
+            assert torch.equal(a, b), (mode, L, rep)
+    g.close()
+    ix.close()
