@@ -153,7 +153,7 @@ def main():
     seeds = None
     if args.seeds > 1:
         t = time.time()
-        seeds = N.pick_seeds(base_d, args.seeds)
+        seeds = N.pick_seeds(ix, args.seeds)
         log("%d k-means seeds in %.2fs" % (len(seeds), time.time() - t))
     tune_table = None
     if args.fast == "auto":

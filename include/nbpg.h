@@ -159,6 +159,15 @@ int nbpg_index_repair(nbpg_index *idx, uint32_t sp, uint64_t *out_isolated);
 int nbpg_index_info(const nbpg_index *idx, int *dtype, uint64_t *n, uint32_t *dims, uint32_t *R,
                     uint32_t *row_stride);
 
+/* ---- start points for the search's get_sp (check_nn_recall.h:282-297,
+ * seeded into the beam as query_params.h:206-210 does): Lloyd k-means with
+ * `count` centres (<= 256) over `sample` rows of the index drawn with
+ * `seed`, `iters` steps, then the row nearest each centre.  Deterministic
+ * (integer centre sums, FP64 distances, lowest index on ties).  Writes the
+ * distinct ids, ascending, to host memory; *out_count <= count of them. */
+int nbpg_pick_seeds(const nbpg_index *idx, uint32_t count, uint32_t sample, uint32_t iters,
+                    uint64_t seed, uint32_t *h_out_ids, uint32_t *out_count);
+
 /* ---- distances: Point::distance / euclidean_distance (point.h:35-120, 303-314)
  * out[i] = dist(query, row ids[i]) for i < n; d_query holds `dims` bytes. */
 int nbpg_l2_batch(const nbpg_index *idx, const void *d_query, const uint32_t *d_ids, uint64_t n,

@@ -90,6 +90,7 @@ _SIGS = {
                                        C.POINTER(QueryParams), I32, C.POINTER(FastConfig),
                                        VP, VP, VP]),
     "nbpg_search_graph_launch": (I32, [VP, VP]),
+    "nbpg_pick_seeds": (I32, [VP, U32, U32, U32, U64, VP, C.POINTER(U32)]),
     "nbpg_search_graph_destroy": (I32, [VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
                           C.POINTER(FastConfig), VP, VP, VP, VP]),
@@ -503,36 +504,15 @@ def pca_stats(x_dev, cov=True):
     return mean, c
 
 
-def _as_float_rows(rows, dtype):
-    """Rows of raw bytes (uint8 tensor) as the float vectors their distance
-    sees: u8 / i8 values, or the two 16-step nibbles of each packed byte."""
-    torch = _torch()
-    if dtype == I8:
-        return rows.view(torch.int8).float()
-    if dtype in (U4X2, I4X2):
-        r = rows.to(torch.int32)
-        return torch.stack([(r & 0xF0), (r << 4) & 0xFF], dim=2).reshape(rows.shape[0], -1).float()
-    return rows.float()
-
-
-def pick_seeds(base_dev, count=64, dtype=U8, sample=65536, iters=10, seed=0):
-    """FAST-mode start nodes spread over the data: Lloyd k-means with `count`
-    centres on a sample of rows, then the row nearest each centre.  Returns
-    distinct uint32 ids (at most count)."""
-    torch = _torch()
-    n = base_dev.shape[0]
-    g = torch.Generator().manual_seed(seed)
-    idx = torch.randperm(n, generator=g)[:min(sample, n)].to(base_dev.device)
-    x = _as_float_rows(base_dev[idx], dtype)
-    c = x[:count].clone()
-    for _ in range(iters):
-        lab = torch.cdist(x, c).argmin(1)
-        s = torch.zeros_like(c).index_add_(0, lab, x)
-        m = torch.bincount(lab, minlength=c.shape[0]).clamp(min=1).unsqueeze(1).float()
-        c = torch.where(m > 0, s / m, c)
-    near = torch.cdist(c, x).argmin(1)
-    ids = idx[near].cpu().numpy().astype(np.uint32)
-    return np.unique(ids)
+def pick_seeds(index, count=64, sample=65536, iters=10, seed=0):
+    """FAST-mode start nodes spread over the index's rows (nbpg_pick_seeds:
+    k-means with `count` centres on a sample, then the row nearest each
+    centre).  Returns distinct uint32 ids, ascending (at most count)."""
+    out = np.zeros(count, np.uint32)
+    m = U32()
+    _check(lib().nbpg_pick_seeds(index.h, count, sample, iters, seed, out.ctypes.data,
+                                 C.byref(m)), "nbpg_pick_seeds")
+    return out[:m.value].copy()
 
 
 def pack_cands(ids, dists):

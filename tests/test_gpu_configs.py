@@ -142,7 +142,7 @@ def test_c2_reference_search(c2):
 def test_c2_fast_search(c2, fast, L):
     """The headline search: 10k queries, 64 k-means seeds, one launch."""
     N = c2["N"]
-    seeds = N.pick_seeds(c2["base_d"], 64)
+    seeds = N.pick_seeds(c2["ix"], 64)
     ids, d, st = N.search(c2["ix"], c2["qd"], L, 10, mode=N.SEARCH_FAST, fast=fast,
                           start_points=seeds)
     ids, d = ids.cpu().numpy().view(np.uint32), d.cpu().numpy()
