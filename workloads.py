@@ -324,6 +324,83 @@ def run_pca(args, log):
     return res
 
 
+# ---------------------------------------------------------------------- refgraph
+def run_refgraph(args, log):
+    """The headline search (1M x 128 u8, 10k queries, k = 10) on the NBPG
+    graph built by the reference's own Vamana binary on the host cores
+    (README.md:96 flags: --R 64 --num_split 4 --num_passes 2
+    --dynamic_prune --min_alpha 1.05; L 70, VL 140, max_alpha 1.1 by
+    default, Vamana.cpp:24-50), compiled with its own optimisation flags,
+    written as a .graph file and loaded by nbpg_index_load_graph; beside it
+    the device-built graph of the same flags.  FAST and plain QPS at the
+    smallest L reaching recall@10 >= 0.95 on each graph."""
+    torch = _torch()
+    import oracle as O
+    import nbpg_amd as N
+    from nbpg_amd import datagen as D
+    N.init(0)
+    n, nq = args.n or 1_000_000, args.nq or 10_000
+    base, q = D.workload_1m(n, nq)
+    _, isa = O.host_cpu()
+    exe = os.path.join(ROOT, "oracle", "_ref", "Vamana_fast_%s" % isa) if isa else None
+    if exe is None or not os.path.exists(exe):
+        exe = os.path.join(ROOT, "oracle", "_ref", "Vamana")
+    if not os.path.exists(exe):
+        raise SystemExit("the reference's Vamana binary is not built (oracle/Makefile ref)")
+    tmp = tempfile.mkdtemp()
+    write_bin(os.path.join(tmp, "b.u8bin"), base)
+    t = time.time()
+    proc = subprocess.Popen([exe, "--quant_base_path", os.path.join(tmp, "b.u8bin"),
+                             "--index_path", os.path.join(tmp, "ref"), "--R", "64",
+                             "--num_split", "4", "--num_passes", "2", "--dynamic_prune",
+                             "--min_alpha", "1.05", "--rebuild"],
+                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    for line in proc.stdout:  # the binary's own progress lines keep the run visible
+        log("ref Vamana:", line.rstrip()[:160])
+    if proc.wait() != 0:
+        raise SystemExit("reference Vamana failed")
+    ref_build_s = time.time() - t
+    log("reference build %.1fs (%s)" % (ref_build_s, os.path.basename(exe)))
+    bd = N.to_device(base)
+    gi, gd = N.compgt(bd, torch.from_numpy(q).cuda(), 100, N.U8, N.GT_MFMA)
+    gt, gtd = gi.cpu().numpy().view(np.uint32), gd.cpu().numpy()
+    qd = N.to_device(pad_rows(q, 128))
+    res = {"workload": "refgraph", "n": n, "nq": nq, "reference_build_s": round(ref_build_s, 1),
+           "reference_binary": os.path.basename(exe), "graphs": []}
+    for name in ("reference", "device"):
+        ix = N.Index(N.U8, n, 128, 64)
+        ix.set_points(base)
+        if name == "reference":
+            ix.load_graph(os.path.join(tmp, "ref.graph"))
+            build_s = ref_build_s
+        else:
+            t = time.time()
+            N.build(ix, 64, 70, VL=140, alpha=1.1, min_alpha=1.05, passes=2, num_split=4,
+                    dynamic_prune=True, sp_seed=1)
+            torch.cuda.synchronize()
+            build_s = time.time() - t
+        rows = ix.get_graph()
+        seeds = N.pick_seeds(ix, 64)
+        row = {"graph": name, "build_s": round(build_s, 2),
+               "avg_degree": round(float(rows[:, 0].mean()), 2)}
+        for mode, tag, sp in ((N.SEARCH_FAST, "fast", seeds), (N.SEARCH_REF, "plain", None)):
+            for L in (10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128):
+                out = N.search(ix, qd, L, 10, mode=mode, start_points=sp,
+                               fast=(2, 1, 10, 0, 0))
+                r = recall(out[0].cpu().numpy().view(np.uint32), gt, gtd, 10)
+                if r >= 0.95:
+                    break
+            reps = 10 if mode == N.SEARCH_FAST else 3
+            ms = events_ms(lambda: N.search(ix, qd, L, 10, mode=mode, start_points=sp,
+                                            fast=(2, 1, 10, 0, 0), out=out), reps)
+            row[tag] = {"L": L, "recall@10": round(r, 4), "qps": round(nq / ms * 1e3, 1)}
+        row["fast_over_plain"] = round(row["fast"]["qps"] / row["plain"]["qps"], 2)
+        log("refgraph", row)
+        res["graphs"].append(row)
+        ix.close()
+    return res
+
+
 # ---------------------------------------------------------------------- uneven
 def run_uneven(args, log):
     """1M x 128 u8 NBPG graph: batches of 1..4096 queries with per-query L
