@@ -48,7 +48,12 @@ def main():
                     help="waves_per_group,expand_width,filter_log2,groups,row_split or 'auto'")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches instead of the library's captured search graph")
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=6)
+    ap.add_argument("--batches", type=int, default=12,
+                    help="distinct query batches searched in turn by the timed steps")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams the steps are issued on round-robin (0 = measured: the "
+                         "autotuner's choice, or the best of 1..4)")
     ap.add_argument("--no-entry0", action="store_true",
                     help="skip timing the FAST path from the single entry node 0 (profiling runs)")
     ap.add_argument("--seeds", type=int, default=64,
@@ -146,8 +151,32 @@ def main():
     log("device ground truth top-100: MFMA %.3fs, VALU %.3fs (identical); reference rule %.3fs,"
         " %d queries differ from the exact top-100" % (gt_s, gt_plain_s, gt_ref_s, gt_ref_differs))
 
-    def recall_at(ids, g):
-        return AT.recall_with_ties(ids, g, gtd[:len(g)], 10)
+    # the timed steps search B distinct query batches in turn (batch 0 is
+    # the workload's own query set, the others fresh draws of the same
+    # mixture), so no batch is searched twice in a row and batches that run
+    # at the same time differ; recall is over all of them
+    B = max(1, args.batches)
+    nq = q.shape[0]
+    q_host = [q] + [D.workload_1m_queries(nq, b, q.shape[1]) for b in range(1, B)]
+    q_ds = [q_d]
+    for qq in q_host[1:]:
+        qp_ = np.zeros((nq, stride), np.uint8)
+        qp_[:, :qq.shape[1]] = qq
+        q_ds.append(N.to_device(qp_))
+    gts, gtds = [gt], [gtd]
+    for qq in q_host[1:]:
+        g_i, g_d = N.compgt(base_d, torch.from_numpy(qq).cuda(), 100, N.U8, N.GT_MFMA)
+        gts.append(g_i.cpu().numpy().view(np.uint32))
+        gtds.append(g_d.cpu().numpy())
+    log("%d query batches of %d, ground truth for each" % (B, nq))
+
+    def recall_all(runner, L):
+        rs = []
+        for b in range(B):
+            ids = runner(L, batch=b)[0]
+            torch.cuda.synchronize()
+            rs.append(AT.recall_with_ties(ids.cpu().numpy().view(np.uint32), gts[b], gtds[b], 10))
+        return float(np.mean(rs))
 
     # FAST-mode start nodes: 64 k-means seeds over the base (the plain path
     # keeps the reference's single entry, node 0)
@@ -157,82 +186,135 @@ def main():
         seeds = N.pick_seeds(ix, args.seeds)
         log("%d k-means seeds in %.2fs" % (len(seeds), time.time() - t))
     tune_table = None
+    streams = args.streams
     if args.fast == "auto":
         t = time.time()
         best, tune_table = AT.tune(ix, q_d, gt, gtd, k=10, target=args.target, log=log,
-                                   start_points=seeds)
+                                   start_points=seeds, q_batches=q_ds)
         fast = best["config"]
-        log("autotuner picked %s at L=%d in %.1fs" % (fast, best["L"], time.time() - t))
+        log("autotuner picked %s at L=%d, %d streams in %.1fs"
+            % (fast, best["L"], best["streams"], time.time() - t))
         if not args.L:
             args.L = best["L"]
+        if not streams:
+            streams = best["streams"]
     else:
         fast = tuple(int(x) for x in args.fast.split(","))
-    sweep = [args.L] if args.L else [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160,
-                                        200, 256]
+    full_sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
 
-    def run_fast(L, out=None):
-        return N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, out=out,
+    def run_fast(L, out=None, batch=0):
+        return N.search(ix, q_ds[batch], L, 10, mode=N.SEARCH_FAST, fast=fast, out=out,
                         start_points=seeds)
 
-    def run_fast_entry0(L, out=None):
-        return N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, out=out)
+    def run_fast_entry0(L, out=None, batch=0):
+        return N.search(ix, q_ds[batch], L, 10, mode=N.SEARCH_FAST, fast=fast, out=out)
 
-    def run_plain(L, out=None):
-        return N.search(ix, q_d, L, 10, mode=N.SEARCH_REF, cut=1.35, out=out)
+    def run_plain(L, out=None, batch=0):
+        return N.search(ix, q_ds[batch], L, 10, mode=N.SEARCH_REF, cut=1.35, out=out)
 
-    def pick(runner, name):
-        for L in sweep:
-            ids, _, st = runner(L)
-            torch.cuda.synchronize()
-            r = recall_at(ids.cpu().numpy().view(np.uint32), gt)
-            log("%s L=%d recall@10 %.4f" % (name, L, r))
+    def pick(runner, name, start=0):
+        r = 0.0
+        for L in [x for x in full_sweep if x >= start] or [start]:
+            r = recall_all(runner, L)
+            log("%s L=%d recall@10 %.4f (%d batches)" % (name, L, r, B))
             if r >= args.target:
-                return L, r, st.cpu().numpy()
-        return sweep[-1], r, st.cpu().numpy()
+                return L, r
+        return L, r
 
-    L_fast, rec_fast, st_fast = pick(run_fast, "fast")
-    sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
-    L_plain, rec_plain, st_plain = pick(run_plain, "plain")
+    L_fast, rec_fast = pick(run_fast, "fast", args.L)
+    L_plain, rec_plain = pick(run_plain, "plain")
     cmp_e0 = seeds is not None and not args.no_entry0
-    L_e0, rec_e0, _ = pick(run_fast_entry0, "fast, entry node 0") if cmp_e0 \
-        else (L_fast, rec_fast, None)
+    L_e0, rec_e0 = pick(run_fast_entry0, "fast, entry node 0") if cmp_e0 else (L_fast, rec_fast)
 
-    def timed(runner, L, steps, warmup, graph_of=None):
-        out = runner(L)
-        step = lambda: runner(L, out)  # noqa: E731
-        if graph_of is not None:
-            # one step captured in a HIP graph by the library itself
-            # (nbpg_search_graph_*), replayed per step on torch's stream
-            g = graph_of(L)
-            step = g.replay
-        for _ in range(warmup):
-            step()
+    def launchers(runner, L, S):
+        """launch(i) searches batch i % B into its own output set (step i
+        runs on stream i % S; steps that share a set share a stream)"""
+        M = AT.output_sets(B, S)
+        outs = [runner(L, batch=j % B) for j in range(M)]
+        return (lambda i: runner(L, outs[i % M], batch=i % B)), outs
+
+    def graph_launchers(L, S, entry0=False):
+        # each batch's search captured by the library as a HIP graph
+        # (nbpg_search_graph_*), one graph per output set
+        M = AT.output_sets(B, S)
+        gs = [N.SearchGraph(ix, q_ds[j % B], L, 10, mode=N.SEARCH_FAST, fast=fast,
+                            start_points=None if entry0 else seeds) for j in range(M)]
+        return (lambda i: gs[i % M].replay()), gs
+
+    def pipelined(launch, S, steps, warmup, per_batch=False):
+        """steps issued round-robin over S streams, bracketed by device
+        synchronisations; returns (host seconds, event-timed ms of the whole
+        region, per-step ms on each step's own stream or None)"""
+        ss = [torch.cuda.Stream() for _ in range(S)]
+        cur = torch.cuda.current_stream()
+
+        def issue(n, ev=None):
+            for i in range(n):
+                with torch.cuda.stream(ss[i % S]):
+                    if ev is not None:
+                        ev[i][0].record()
+                    launch(i)
+                    if ev is not None:
+                        ev[i][1].record()
+        issue(warmup)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(steps)]
-        torch.cuda.synchronize()
+              for _ in range(steps)] if per_batch else None
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for st in ss:
+            st.wait_event(e0)
         t0 = time.perf_counter()
-        for a, b in ev:
-            a.record()
-            step()
-            b.record()
+        issue(steps, ev)
+        for st in ss:
+            cur.wait_stream(st)
+        e1.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        per = np.array([a.elapsed_time(b) for a, b in ev])
-        return wall, per
+        per = np.array([a.elapsed_time(b) for a, b in ev]) if ev else None
+        return wall, e0.elapsed_time(e1), per
 
-    def graph_fast(L):
-        return N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast, start_points=seeds)
-
-    def graph_fast_entry0(L):
-        return N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast)
-
-    wall, per = timed(run_fast, L_fast, args.steps, args.warmup,
-                      None if args.no_graph else graph_fast)
-    nq = q.shape[0]
-    qps = nq * args.steps / wall
-    log("fast L=%d: %.0f QPS, step p50 %.3f ms p99 %.3f ms" %
-        (L_fast, qps, np.percentile(per, 50), np.percentile(per, 99)))
+    if not streams:
+        # --fast given: pick the stream count by the same measurement
+        best_s = None
+        for S in (1, 2, 3, 4):
+            launch, _ = launchers(run_fast, L_fast, S)
+            w, _, _ = pipelined(launch, S, 24, S)
+            log("fast L=%d, %d streams: %.0f QPS" % (L_fast, S, nq * 24 / w))
+            if best_s is None or w < best_s[1]:
+                best_s = (S, w)
+        streams = best_s[0]
+    S = streams
+    K = args.steps
+    # the timed steps: eager calls and replays of the library's graphs, each
+    # over S streams; the faster is the headline (both are reported)
+    launch_e, outs_e = launchers(run_fast, L_fast, S)
+    wall_e, region_e, _ = pipelined(launch_e, S, K, args.warmup)
+    qps_e = nq * K / wall_e
+    log("fast L=%d eager, %d streams: %.0f QPS (%.4f ms per step)" % (L_fast, S, qps_e,
+                                                                       wall_e / K * 1e3))
+    qps_g, wall_g, region_g = None, None, None
+    if not args.no_graph:
+        launch_g, gs = graph_launchers(L_fast, S)
+        wall_g, region_g, _ = pipelined(launch_g, S, K, args.warmup)
+        qps_g = nq * K / wall_g
+        log("fast L=%d graph replays, %d streams: %.0f QPS" % (L_fast, S, qps_g))
+        for j in range(min(len(gs), len(outs_e))):
+            if not (torch.equal(gs[j].ids, outs_e[j][0]) and torch.equal(gs[j].dists, outs_e[j][1])):
+                raise SystemExit("graph replay and eager results differ (output set %d)" % j)
+    use_graph = qps_g is not None and qps_g > qps_e
+    wall, region, qps = (wall_g, region_g, qps_g) if use_graph else (wall_e, region_e, qps_e)
+    launch_best = launch_g if use_graph else launch_e
+    # per-batch latency: under the same pipelined load, and one batch at a
+    # time on one stream; throughput with one stream beside it
+    _, _, per = pipelined(launch_best, S, K, args.warmup, per_batch=True)
+    launch_1, _ = launchers(run_fast, L_fast, 1)
+    _, _, per_alone = pipelined(launch_1, 1, K, args.warmup, per_batch=True)
+    wall_1, _, _ = pipelined(launch_1, 1, K, args.warmup)
+    qps_1 = nq * K / wall_1
+    log("fast: batch latency p50 %.3f / p99 %.3f ms with %d streams, %.3f / %.3f ms alone; "
+        "one stream %.0f QPS" % (np.percentile(per, 50), np.percentile(per, 99), S,
+                                 np.percentile(per_alone, 50), np.percentile(per_alone, 99), qps_1))
     # end to end: queries from pinned host memory, search, ids and distances
     # back to pinned host memory, one synchronous batch per step (host clock)
     qh = torch.from_numpy(qpad).pin_memory()
@@ -251,7 +333,7 @@ def main():
     for _ in range(args.warmup):
         e2e_step()
     e2e = []
-    for _ in range(args.steps):
+    for _ in range(K):
         t0 = time.perf_counter()
         e2e_step()
         e2e.append(time.perf_counter() - t0)
@@ -262,37 +344,51 @@ def main():
         % (nq / (e2e.mean() * 1e-3), np.percentile(e2e, 50), np.percentile(e2e, 99)))
     e0_qps = qps
     if cmp_e0:
-        w0, _ = timed(run_fast_entry0, L_e0, max(5, args.steps // 2), 2,
-                      None if args.no_graph else graph_fast_entry0)
-        e0_qps = nq * max(5, args.steps // 2) / w0
+        K0 = max(6, K // 2)
+        l0, _ = (graph_launchers(L_e0, S, entry0=True) if use_graph
+                 else launchers(run_fast_entry0, L_e0, S))
+        w0, _, _ = pipelined(l0, S, K0, 2)
+        e0_qps = nq * K0 / w0
         log("fast with the single entry node 0, L=%d: %.0f QPS" % (L_e0, e0_qps))
-    pwall, pper = timed(run_plain, L_plain, args.plain_steps, 1)
+    lp, _ = launchers(run_plain, L_plain, S)
+    pwall, _, _ = pipelined(lp, S, args.plain_steps, 1)
     plain_qps = nq * args.plain_steps / pwall
-    log("plain L=%d: %.0f QPS" % (L_plain, plain_qps))
+    log("plain L=%d, %d streams: %.0f QPS" % (L_plain, S, plain_qps))
 
     # roofline of the fast kernel: algorithmic bytes of one launch from each
     # query's own counters -- expansions, distance computations and the
     # adjacency entries its expanded nodes hold (NBPG_SEARCH_EDGES):
     #   B_q = 4 * (visited + edges) + D * dist_cmps + D
-    # The seed rows' distances (every query starts with one per seed) hit
-    # the same L2-resident rows for every query; the headline figure leaves
-    # them out and the figure with them is reported beside it.
-    e_ids, _, st3 = N.search(ix, q_d, L_fast, 10, mode=N.SEARCH_FAST, fast=fast,
-                             start_points=seeds, edges=True)
-    if not np.array_equal(e_ids.cpu().numpy(), run_fast(L_fast)[0].cpu().numpy()):
-        raise SystemExit("FAST results differ between two calls")
-    st3 = st3.cpu().numpy().astype(np.float64)
-    vis, cmps, edges = st3[:, 0], st3[:, 1], st3[:, 2]
+    # averaged over the batches.  The seed rows' distances (every query
+    # starts with one per seed) hit the same L2-resident rows for every
+    # query; the headline figure leaves them out and the figure with them is
+    # reported beside it.
     dbytes = base.shape[1]
     nseed = len(seeds) if seeds is not None else 1
-    alg_with_seeds = float(4 * (vis + edges).sum() + (cmps * dbytes).sum() + nq * dbytes)
+    st_all = []
+    for b in range(B):
+        e_ids, _, st3 = N.search(ix, q_ds[b], L_fast, 10, mode=N.SEARCH_FAST, fast=fast,
+                                 start_points=seeds, edges=True)
+        if not torch.equal(e_ids, run_fast(L_fast, batch=b)[0]):
+            raise SystemExit("FAST results differ between two calls")
+        st_all.append(st3.cpu().numpy().astype(np.float64))
+    st3 = np.concatenate(st_all)
+    vis, cmps, edges = st3[:, 0], st3[:, 1], st3[:, 2]
+    alg_with_seeds = float(4 * (vis + edges).sum() + (cmps * dbytes).sum() + len(st3) * dbytes) / B
     alg_bytes = alg_with_seeds - float(nq * nseed * dbytes)
-    kernel_ms = float(per.mean())
+    # the timed region by HIP events (first step's stream start to the last
+    # stream's end), per step: launches overlap, so this is the device time
+    # each batch costs
+    kernel_ms = region / K
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     achieved_seeds = alg_with_seeds / (kernel_ms * 1e-3) / 1e9
-    log("roofline: %.1f MB per launch without the seed rows (%.1f MB with), %.0f / %.0f GB/s"
-        % (alg_bytes / 1e6, alg_with_seeds / 1e6, achieved, achieved_seeds))
+    alone_ms = float(per_alone.mean())
+    log("roofline: %.1f MB per launch without the seed rows (%.1f MB with), %.0f / %.0f GB/s; "
+        "one launch alone %.3f ms, %.0f GB/s" % (alg_bytes / 1e6, alg_with_seeds / 1e6, achieved,
+                                                 achieved_seeds, alone_ms,
+                                                 alg_bytes / (alone_ms * 1e-3) / 1e9))
 
+    traffic, traffic_cfg = measured_traffic(base.shape[0], nq, base.shape[1], L_fast, fast)
     cpu = None
     if not args.no_cpu:
         global GTD
@@ -304,41 +400,56 @@ def main():
         "value": round(qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(wall / K * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded 1000-cluster Gaussian mixture, heavy-tailed dims)",
-        "config": {"workload": "1M x 128 u8 base, 10k queries, NBPG graph R=64 built on device, "
-                               "k=10, fast persistent search", "n": base.shape[0], "nq": nq,
-                   "L": L_fast, "fast_config": list(fast), "recall@10": round(rec_fast, 4),
+        "data": "synthetic (seeded 1000-cluster Gaussian mixture, heavy-tailed dims); %d distinct "
+                "query batches searched in turn" % B,
+        "config": {"workload": "1M x 128 u8 base, 10k-query batches, NBPG graph R=64 built on "
+                               "device, k=10, fast persistent search", "n": base.shape[0],
+                   "nq": nq, "query_batches": B, "L": L_fast, "fast_config": list(fast),
+                   "recall@10": round(rec_fast, 4),
                    "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
                    else "node 0",
-                   "hipgraph": not args.no_graph,
+                   "streams": S, "launch": "hipgraph" if use_graph else "eager",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(base.shape[0], nq, base.shape[1], L_fast, fast),
+                     "traffic": traffic, "traffic_profiled_config": traffic_cfg,
                      "algorithmic_bytes": round(alg_bytes), "kernel_ms": round(kernel_ms, 5),
-                     "bytes": "sum over queries of 4*(visited + adjacency entries of the "
-                              "expanded nodes) + 128*(dist_cmps - seed rows) + 128, from the "
-                              "kernel's own per-query counters",
+                     "kernel_ms_is": "event-timed region / steps (launches overlap on %d streams)"
+                                     % S,
+                     "bytes": "mean over the batches of the sum over queries of 4*(visited + "
+                              "adjacency entries of the expanded nodes) + 128*(dist_cmps - seed "
+                              "rows) + 128, from the kernel's own per-query counters",
                      "with_seed_rows": {"algorithmic_bytes": round(alg_with_seeds),
                                         "achieved": round(achieved_seeds, 1),
-                                        "frac": round(achieved_seeds / PEAK_HBM_GBS, 4)}},
+                                        "frac": round(achieved_seeds / PEAK_HBM_GBS, 4)},
+                     "one_launch_alone": {"kernel_ms": round(alone_ms, 5),
+                                          "achieved": round(alg_bytes / (alone_ms * 1e-3) / 1e9,
+                                                            1),
+                                          "frac": round(alg_bytes / (alone_ms * 1e-3) / 1e9
+                                                        / PEAK_HBM_GBS, 4)}},
         "cpu_baseline": cpu,
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
+        "step_ms_is": "one batch's device time on its stream, %d streams in flight" % S,
+        "alone": {"p50_ms": round(float(np.percentile(per_alone, 50)), 4),
+                  "p99_ms": round(float(np.percentile(per_alone, 99)), 4),
+                  "qps_one_stream": round(qps_1, 1)},
+        "pipelined": {"eager_qps": round(qps_e, 1),
+                      "graph_qps": round(qps_g, 1) if qps_g is not None else None},
         "end_to_end": {"qps": round(nq / (float(e2e.mean()) * 1e-3), 1),
                        "p50_ms": round(float(np.percentile(e2e, 50)), 4),
                        "p99_ms": round(float(np.percentile(e2e, 99)), 4),
                        "what": "query rows from pinned host memory, search, ids and distances "
                                "back to pinned host memory, synchronised per batch (host clock)"},
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
-                  "speedup_fast_over_plain": round(qps / plain_qps, 2)},
+                  "streams": S, "speedup_fast_over_plain": round(qps / plain_qps, 2)},
         "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)}
         if cmp_e0 else None,
         "build_s": round(build_s, 2),
@@ -350,8 +461,8 @@ def main():
         "mean_dist_cmps": round(float(cmps.mean()), 1),
         "mean_expanded_degree": round(float(edges.sum() / max(vis.sum(), 1.0)), 2),
         "graph_avg_degree": round(avg_deg, 2),
-        "autotune": [{"config": list(r["config"]), "L": r["L"], "qps": round(r["qps"], 1)}
-                     for r in (tune_table or [])],
+        "autotune": [{"config": list(r["config"]), "L": r["L"], "streams": r["streams"],
+                      "qps": round(r["qps"], 1)} for r in (tune_table or [])],
     }
     print(json.dumps(res), flush=True)
 
@@ -362,19 +473,26 @@ GTD = None
 def measured_traffic(n, nq, dims, L, fast):
     """HBM bytes per FAST launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries of this workload (calibrated against a 1 GiB copy),
-    one per profiled launch shape; None when this run's workload or launch
-    shape was not profiled."""
+    one per profiled launch shape: (bytes, profiled shape).  A shape that
+    differs only in its group count takes the bytes of the profiled one (the
+    grid size changes occupancy, not what a query reads: 2048 and 3072
+    groups fetched within 0.3% of each other).  (None, None) when this
+    workload or kernel shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "search_reg_kernel_1m_u8_L16.json")
     try:
         with open(path) as f:
             prof = json.load(f)
     except OSError:
-        return None
-    for p in prof["runs"]:
-        if (p["n"], p["nq"], p["dims"], p["L"], list(p["fast_config"])) == \
-                (n, nq, dims, L, list(fast)):
-            return p["traffic_bytes_per_launch"]
-    return None
+        return None, None
+    same = [p for p in prof["runs"] if (p["n"], p["nq"], p["dims"], p["L"]) == (n, nq, dims, L)]
+    for p in same:
+        if list(p["fast_config"]) == list(fast):
+            return p["traffic_bytes_per_launch"], list(fast)
+    for p in same:
+        c = list(p["fast_config"])
+        if c[:3] + c[4:] == list(fast)[:3] + list(fast)[4:]:
+            return p["traffic_bytes_per_launch"], c
+    return None, None
 
 
 def cpu_baseline(base, q, rows, gt, L, seconds):

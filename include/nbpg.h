@@ -6,9 +6,14 @@
  * optional hipStream_t (passed as void*, NULL = the legacy default stream),
  * and returns an int status (NBPG_OK or a negative code; never aborts).
  * nbpg_last_error() gives the message of the calling thread's last failure.
- * Calls on one index share its work counter and workspace: launches on one
- * index must not overlap in time (issue them on one stream, or synchronise
- * between streams); a server (nbpg_server_*) keeps counters of its own.
+ * Searches on one index (nbpg_search, nbpg_search_uneven, graph replays)
+ * may run at the same time on different streams: each launch takes the
+ * next of the index's 64 work counters (so at most 64 may be in flight on
+ * the device at once), and a search graph owns one.  Calls that use the
+ * index's workspace -- nbpg_search_rerank, NBPG_SEARCH_REF with L beyond
+ * its LDS budget, prune, build, repair -- must not overlap other calls on
+ * the index (issue them on one stream, or synchronise between streams); a
+ * server (nbpg_server_*) keeps counters of its own.
  *
  * Citations are file:line in the reference tree (godpgf/NBPG), naming the
  * interface each entry point stands in for.
@@ -211,10 +216,11 @@ int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq
  * batch.  create makes one eager call (argument checks, workspace growth),
  * then captures the same call; launch replays it on `stream`, reading
  * whatever d_queries holds at that time and writing the same outputs.  The
- * index, buffers and start points are fixed at creation; the FAST kernel's
- * work counter returns itself to zero, so the graph holds no memset.
- * Launches of one graph, and of any search on the same index, must not
- * overlap in time. */
+ * index, buffers and start points are fixed at creation; the graph has its
+ * own FAST work counter, which the kernel returns to zero, so the graph
+ * holds no memset.  Launches of one graph must not overlap each other in
+ * time (they share the counter and the outputs); other searches on the
+ * index may run beside them. */
 typedef struct nbpg_search_graph nbpg_search_graph;
 int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *idx, const void *d_queries,
                              uint64_t nq, uint64_t query_stride, const uint32_t *h_start_points,

@@ -343,13 +343,16 @@ def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100
     return ids, d, st
 
 
-def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None):
+def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None, out=None):
     torch = _torch()
     nq = q_dev.shape[0]
     qstride = q_dev.shape[1] * q_dev.element_size()
-    ids = torch.empty((nq, out_k), dtype=torch.int32, device="cuda")
-    d = torch.empty((nq, out_k), dtype=torch.float32, device="cuda")
-    st = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+    if out is None:
+        ids = torch.empty((nq, out_k), dtype=torch.int32, device="cuda")
+        d = torch.empty((nq, out_k), dtype=torch.float32, device="cuda")
+        st = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+    else:
+        ids, d, st = out
     fc = FastConfig(*fast) if fast is not None else None
     _check(lib().nbpg_search_uneven(index.h, _ptr(q_dev), nq, qstride, _ptr(L_dev), _ptr(k_dev),
                                     max_L, out_k, entry, C.byref(fc) if fc is not None else None,

@@ -209,6 +209,16 @@ __global__ void l2_pairs_kernel(int dtype, const uint8_t *__restrict__ a, const 
     out[i] = (float)s;
 }
 
+// the flag lines, then the work counters (256 bytes each)
+#define INDEX_COUNTER_BYTES (512 + 256 * NBPG_WORK_COUNTERS)
+
+uint32_t *work_counter(const nbpg_index *ix)
+{
+    nbpg_index *m = const_cast<nbpg_index *>(ix);
+    const uint32_t s = __atomic_fetch_add(&m->launch_seq, 1u, __ATOMIC_RELAXED) % NBPG_WORK_COUNTERS;
+    return ix->counter + 128 + 64 * s;
+}
+
 } // namespace nbpg
 
 using namespace nbpg;
@@ -274,7 +284,7 @@ int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, ui
     bool ok = hipMalloc(&ix->vecs, n * ix->stride) == hipSuccess &&
               hipMalloc(&ix->norms, n * 4) == hipSuccess &&
               hipMalloc(&ix->nbr, n * (uint64_t)R * 4) == hipSuccess &&
-              hipMalloc(&ix->deg, n * 4) == hipSuccess && hipMalloc(&ix->counter, 512) == hipSuccess;
+              hipMalloc(&ix->deg, n * 4) == hipSuccess && hipMalloc(&ix->counter, INDEX_COUNTER_BYTES) == hipSuccess;
     if (!ok) {
         nbpg_index_destroy(ix);
         return set_error(NBPG_ENOMEM, "index of %llu rows", (unsigned long long)n);
@@ -283,7 +293,7 @@ int nbpg_index_create(nbpg_index **out, int dtype, uint64_t n, uint32_t dims, ui
     HIP_TRY(hipMemset(ix->norms, 0, n * 4));
     HIP_TRY(hipMemset(ix->nbr, 0xFF, n * (uint64_t)R * 4));
     HIP_TRY(hipMemset(ix->deg, 0, n * 4));
-    HIP_TRY(hipMemset(ix->counter, 0, 512));
+    HIP_TRY(hipMemset(ix->counter, 0, INDEX_COUNTER_BYTES));
     *out = ix;
     return NBPG_OK;
 }

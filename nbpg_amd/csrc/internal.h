@@ -32,9 +32,12 @@ struct nbpg_index {
     void *ws; // grow-only workspace for search/prune scratch
     size_t ws_bytes;
     // 128 u32 of device flags and counters, each user on its own 128-byte
-    // line: [0] FAST work queue, [32] graph upload check, [40] prune flag,
-    // [48] FAST exit count, [64] BFS / repair counts
+    // line: [32] graph upload check, [40] prune flag, [64] BFS / repair
+    // counts; then, from [128], NBPG_WORK_COUNTERS work counters of the
+    // persistent searches (64 u32 each: the hand-out count, and at +48 the
+    // exit count)
     uint32_t *counter;
+    uint32_t launch_seq; // next work counter (host side, atomic)
 };
 
 namespace nbpg {
@@ -66,6 +69,12 @@ extern DevInfo g_dev;
 
 int ensure_init();
 int ensure_ws(nbpg_index *idx, size_t bytes);
+// A persistent search's work counter (zero when idle; the launch's last wave
+// resets it).  Launches take the index's counters in turn, so launches on
+// different streams may run at the same time as long as fewer than
+// NBPG_WORK_COUNTERS of them are in flight on the device at once.
+#define NBPG_WORK_COUNTERS 64
+uint32_t *work_counter(const nbpg_index *ix);
 // DFS::repair_isolate_node on the resident graph (graph_io.hip)
 int device_repair(nbpg_index *ix, uint32_t sp, uint64_t *out_isolated);
 // rerank of candidate rows (rerank.hip); raw query rows at q_stride bytes

@@ -563,10 +563,13 @@ static int check_search(const nbpg_index *ix, const void *d_queries, uint64_t nq
     return NBPG_OK;
 }
 
-int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,
-                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
-                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
-                uint32_t *d_out_stats, void *stream)
+// nbpg_search's work; counter: the FAST launch's work counter (NULL: the
+// index's next one)
+static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq,
+                        uint64_t query_stride, const uint32_t *h_start_points, uint32_t sp_num,
+                        const nbpg_query_params *qp, int mode, const nbpg_fast_config *fast,
+                        uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
+                        void *stream, uint32_t *counter)
 {
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
     const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
@@ -583,7 +586,17 @@ int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                           nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
     return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points, sp_num,
                        qp->L, out_k, nullptr, nullptr, qp->L, out_k, qp->limit, fast, d_out_ids,
-                       d_out_dists, d_out_stats, ix->counter, as_stream(stream), edges ? 3 : 2);
+                       d_out_dists, d_out_stats, counter ? counter : work_counter(ix),
+                       as_stream(stream), edges ? 3 : 2);
+}
+
+int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,
+                const uint32_t *h_start_points, uint32_t sp_num, const nbpg_query_params *qp,
+                int mode, const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
+                uint32_t *d_out_stats, void *stream)
+{
+    return search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
+                        d_out_ids, d_out_dists, d_out_stats, stream, nullptr);
 }
 
 // nbpg_search_rerank's work; fixed != NULL: every scratch buffer comes from
@@ -635,7 +648,7 @@ static int search_rerank_impl(const nbpg_index *ix, const void *d_queries, uint6
     // so the scratch needs no clearing between calls
     NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
                          sp_num, qp->L, qp->L, nullptr, nullptr, qp->L, qp->L, qp->limit, fast,
-                         cand, nullptr, d_out_stats, ix->counter, s));
+                         cand, nullptr, d_out_stats, work_counter(ix), s));
     return rerank_launch(raw_dtype, ix->mips, d_raw_base, ix->n, raw_dims, d_raw_queries, raw_query_stride,
                          nq, cand, qp->L, qp->L, qp->k, rerank_factor, d_out_ids, d_out_dists, s);
 }
@@ -689,7 +702,7 @@ int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
     if (nq >= 0xFFFFFFFFull)
         return set_error(NBPG_EUNSUP, "nq too large");
     return search_fast(ix, d_queries, nq, query_stride, entry, nullptr, 0, 0, 0, d_L, d_k, max_L,
-                       out_k, ix->n, fast, d_out_ids, d_out_dists, d_out_stats, ix->counter,
+                       out_k, ix->n, fast, d_out_ids, d_out_dists, d_out_stats, work_counter(ix),
                        as_stream(stream));
 }
 
@@ -702,6 +715,7 @@ int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
 struct nbpg_search_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint32_t *counter = nullptr; // the captured launch's own work counter
 };
 
 extern "C" {
@@ -723,15 +737,24 @@ int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *ix, cons
     NBPG_TRY(nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
                          d_out_ids, d_out_dists, d_out_stats, nullptr));
     HIP_TRY(hipDeviceSynchronize());
-    hipStream_t s = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     nbpg_search_graph *g = new nbpg_search_graph();
+    // the graph's own work counter, so replays never share one with other
+    // launches on the index
+    if (hipMalloc(&g->counter, 256) != hipSuccess || hipMemset(g->counter, 0, 256) != hipSuccess) {
+        nbpg_search_graph_destroy(g);
+        return set_error(NBPG_ENOMEM, "graph work counter");
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        nbpg_search_graph_destroy(g);
+        return set_error(NBPG_EHIP, "capture stream");
+    }
     int rc = NBPG_OK;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         rc = set_error(NBPG_EHIP, "graph capture start");
     } else {
-        rc = nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
-                         d_out_ids, d_out_dists, d_out_stats, s);
+        rc = search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
+                          d_out_ids, d_out_dists, d_out_stats, s, g->counter);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &graph);
         if (rc == NBPG_OK && e != hipSuccess)
@@ -765,6 +788,10 @@ int nbpg_search_graph_destroy(nbpg_search_graph *g)
         hipGraphExecDestroy(g->exec);
     if (g->graph)
         hipGraphDestroy(g->graph);
+    if (g->counter) {
+        hipDeviceSynchronize(); // a replay still running reads it
+        hipFree(g->counter);
+    }
     delete g;
     return NBPG_OK;
 }

@@ -73,6 +73,15 @@ def workload_1m(n=1_000_000, nq=10_000, dims=128):
     return base, queries
 
 
+def workload_1m_queries(nq=10_000, batch=0, dims=128):
+    """Query batch `batch` of the 1M x 128 u8 workload: the same mixture as
+    its base; batch 0 is workload_1m's own query set, later batches are
+    fresh draws."""
+    sc = heavy_tail_scale(dims)
+    return mixture_u8(nq, dims, 1000, 25.0, seed=4 if batch == 0 else 1000 + batch, centers_seed=9,
+                      scale=sc, shift=66.0)
+
+
 def workload_serve(n=10_000_000, nq=10_000, dims=96, seed=21):
     """10M x 96 i8 rows in -127..127: 4096 clusters (centres U[-90, 90]),
     sigma 30.  Drawn on the device (torch's seeded generator) and returned

@@ -115,16 +115,23 @@ def ref_available():
 def host_cpu():
     """(model name, x86-64 ISA level 'v4' / 'v3' / None) of this host, from
     /proc/cpuinfo."""
-    model, flags = "unknown", set()
+    model, flags, ids = "unknown", set(), {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name") and model == "unknown":
+                key = line.split(":", 1)[0].strip()
+                if key == "model name" and model == "unknown":
                     model = line.split(":", 1)[1].strip()
-                elif line.startswith("flags") and not flags:
+                elif key in ("cpu family", "model", "stepping") and key not in ids:
+                    ids[key] = line.split(":", 1)[1].strip()
+                elif key == "flags" and not flags:
                     flags = set(line.split(":", 1)[1].split())
     except OSError:
         pass
+    if ids:
+        # a masked model name (virtual machines) still carries the family
+        model += " (family %s, model %s, stepping %s)" % (
+            ids.get("cpu family", "?"), ids.get("model", "?"), ids.get("stepping", "?"))
     v3 = {"avx2", "fma", "bmi1", "bmi2", "movbe", "f16c", "lzcnt" if "lzcnt" in flags else "abm"}
     v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
     level = "v4" if v3 <= flags and v4 <= flags else ("v3" if v3 <= flags else None)

@@ -646,3 +646,49 @@ def test_search_graph_replay_equals_eager(gpu, small, mode, L):
             assert torch.equal(a, b), (mode, L, rep)
     g.close()
     ix.close()
+
+
+@pytest.mark.parametrize("kind", ["fast", "uneven"])
+def test_concurrent_searches_on_streams(gpu, small, kind):
+    """Searches on one index issued round-robin on three streams, 70 of them
+    (more than the index's 64 work counters, so every counter is reused),
+    plus two search graphs replayed on a fourth stream in between: every
+    launch returns what the same search returns alone."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    seeds = N.pick_seeds(ix, 16)
+    batches = [dev_rows(N, q[i * 50:(i + 1) * 50], ix.row_stride) for i in range(4)]
+    Ls = torch.tensor(np.full(50, 24, np.uint32).view(np.int32), device="cuda")
+    ks = torch.tensor(np.full(50, 10, np.uint32).view(np.int32), device="cuda")
+
+    def run(b, out=None):
+        if kind == "fast":
+            return N.search(ix, batches[b], 24, 10, mode=N.SEARCH_FAST, start_points=seeds,
+                            out=out)
+        return N.search_uneven(ix, batches[b], Ls, ks, 24, 10, out=out)
+    want = []
+    for b in range(4):
+        want.append([t.clone() for t in run(b)])
+        torch.cuda.synchronize()
+    graphs = [N.SearchGraph(ix, batches[b], 24, 10, mode=N.SEARCH_FAST, start_points=seeds)
+              for b in (0, 1)] if kind == "fast" else []
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    outs = [[torch.empty_like(t) for t in want[i % 4]] for i in range(70)]
+    torch.cuda.synchronize()
+    for i in range(70):
+        with torch.cuda.stream(streams[i % 3]):
+            run(i % 4, outs[i])
+        if graphs and i % 10 == 5:
+            with torch.cuda.stream(streams[3]):
+                graphs[(i // 10) % 2].replay()
+    torch.cuda.synchronize()
+    for i in range(70):
+        for a, b in zip(outs[i], want[i % 4]):
+            assert torch.equal(a, b), (kind, i)
+    for j, g in enumerate(graphs):
+        for a, b in zip((g.ids, g.dists, g.stats), want[j]):
+            assert torch.equal(a, b), ("graph", j)
+        g.close()
+    ix.close()
