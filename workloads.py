@@ -72,6 +72,22 @@ def ref_lib():
         return None
 
 
+def ref_timing_lib():
+    """The reference's code for CPU timing: built with its own optimisation
+    flags at this host's ISA level when that build exists, else the parity
+    build.  Returns (library, flags) or (None, None)."""
+    try:
+        import oracle as O
+    except Exception:
+        return None, None
+    R, isa = O.ref_fast()
+    if R is not None:
+        return R, ("-Ofast -fopenmp -fopenmp-simd -funroll-loops -falign-jumps=64 -mcx16 "
+                   "-march=x86-64-%s" % isa)
+    R = O.ref()
+    return R, ("-O3 -fopenmp -ffp-contract=off (parity build)" if R is not None else None)
+
+
 def write_bin(path, a):
     with open(path, "wb") as f:
         np.array([a.shape[0], a.shape[1]], np.uint32).tofile(f)
@@ -304,7 +320,7 @@ def run_pca(args, log):
         ix.close()
     res["search_on_codes"] = searches
     # CPU baseline: the reference's matmul + compress on a bounded sample
-    R = ref_lib()
+    R, cpu_flags = ref_timing_lib()
     if R is not None and not args.no_cpu:
         m = 20_000
         xh = x[:m].cpu().numpy()
@@ -317,10 +333,17 @@ def run_pca(args, log):
         R.nbpgr_compress(y.ctypes.data, m, d_out, 1, 1, out.ctypes.data)
         cpu_s = time.perf_counter() - t
         res["cpu_baseline"] = {"rows_per_s": round(m / cpu_s, 1), "kind": "reference",
-                               "cores": int(R.nbpgr_threads()),
+                               "cores": int(R.nbpgr_threads()), "flags": cpu_flags,
                                "sample": "%d rows, matmul + compress to i8" % m}
-        _, yp = N.pca_project(x[:m], W, b, code=N.CODE_F32, mode=N.PCA_PLAIN)
-        res["plain_fp32_equals_reference_y"] = bool((yp.cpu().numpy() == y).all())
+        # bit parity against the parity build (the timing build may reorder
+        # its float sums)
+        Rp = ref_lib()
+        if Rp is not None:
+            y = np.zeros((m, d_out), np.float32)
+            Rp.nbpgr_matmul(xh.ctypes.data, m, d_in, Wh.ctypes.data, bh.ctypes.data, d_out,
+                            y.ctypes.data)
+            _, yp = N.pca_project(x[:m], W, b, code=N.CODE_F32, mode=N.PCA_PLAIN)
+            res["plain_fp32_equals_reference_y"] = bool((yp.cpu().numpy() == y).all())
     return res
 
 
@@ -511,7 +534,7 @@ def run_uneven(args, log):
           "nodes_per_s": round(P / ms * 1e3, 1),
           "GBs_candidate_rows": round(P * C * (base.shape[1] + 8) / (ms * 1e-3) / 1e9, 1)}
     log("batched prune %.3f ms" % ms)
-    R = ref_lib()
+    R, flags = ref_timing_lib()
     if R is not None and not args.no_cpu:
         c = rec.copy()
         cnt = np.zeros(P, np.uint32)
@@ -524,7 +547,7 @@ def run_uneven(args, log):
         same = int((same_cnt & same_ids).sum())
         pr.update({"identical_lists": same, "all_identical": same == P,
                    "cpu_reference_s": round(cpu_s, 3), "cpu_threads": int(R.nbpgr_threads()),
-                   "speedup_vs_cpu": round(cpu_s * 1e3 / ms, 1)})
+                   "cpu_flags": flags, "speedup_vs_cpu": round(cpu_s * 1e3 / ms, 1)})
         log("prune identical %d / %d, reference %.2fs on %d threads" % (same, P, cpu_s,
                                                                          R.nbpgr_threads()))
     res["robust_prune"] = pr
