@@ -61,6 +61,15 @@ extern "C" {
  * the adjacency entries the query's expanded nodes hold (the sum of their
  * degrees), for byte accounting; d_out_stats then holds nq*3 u32. */
 #define NBPG_SEARCH_EDGES 0x20
+/* OR into nbpg_search's NBPG_SEARCH_FAST mode: write each query's visited
+ * list -- its expanded nodes in expansion order, the reference's
+ * bsCell.visited (beamSearch.h:43-171) as a set -- instead of its top k:
+ * NBPG_VISITED_CAP(L) ids per query (output stride), 0xFFFFFFFF past the
+ * count; d_out_dists is not written.  L <= 256. */
+#define NBPG_SEARCH_VISITED 0x40
+/* visited-list capacity per query: the reference's 8L (beamSearch.h:279),
+ * at most 1024 */
+#define NBPG_VISITED_CAP(L) ((8u * (L)) < 1024u ? (8u * (L)) : 1024u)
 
 /* ---- CompGT modes ---- */
 #define NBPG_GT_PLAIN 0 /* VALU dot4, one query tile per workgroup */
@@ -197,13 +206,16 @@ int nbpg_search(const nbpg_index *idx, const void *d_queries, uint64_t nq, uint6
  * and d_raw_queries (raw_query_stride bytes apart) give the final distances:
  *   rerank_factor > 0: rerank mode -- the first min(k * rerank_factor, fs)
  *     frontier entries are re-scored and the k nearest by (dist, id) kept;
- *   rerank_factor = 0: filtering mode (NBPG_SEARCH_REF only) -- every visited
+ *   rerank_factor = 0: filtering mode -- every visited
  *     entry is re-scored and the first min(k, fs, visited) are kept; its
  *     dist_cmps counter adds those exact distances as the reference does.
  * NBPG_SEARCH_REF returns the reference's IDs (f32 rows: distances summed in
  * index order, point.h:137-146).  NBPG_SEARCH_FAST re-scores the top
- * min(k * rerank_factor, L) <= 1024 of its beam, using the index's workspace
- * (calls on one index must not overlap).  Outputs nq x k. */
+ * min(k * rerank_factor, L) <= 1024 of its beam; with rerank_factor = 0
+ * (filtering) it re-scores the query's visited list (NBPG_SEARCH_VISITED,
+ * up to NBPG_VISITED_CAP(L) entries, L <= 256), keeps the first k by
+ * (dist, id), and counts the re-scored entries in dist_cmps.  FAST uses the
+ * index's workspace (calls on one index must not overlap).  Outputs nq x k. */
 int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq,
                        uint64_t query_stride, int raw_dtype, const void *d_raw_base,
                        uint32_t raw_dims, const void *d_raw_queries, uint64_t raw_query_stride,

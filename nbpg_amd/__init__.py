@@ -20,7 +20,12 @@ CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
 # PCACompress picks the code type from the output extension (PCACompress.cpp:25-37)
 CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
                ".i4bin": CODE_I4}
-SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER, SEARCH_EDGES = 0, 1, 0x10, 0x20
+SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER, SEARCH_EDGES, SEARCH_VISITED = 0, 1, 0x10, 0x20, 0x40
+
+
+def visited_cap(L):
+    """ids per query of a visited list (NBPG_VISITED_CAP): 8L, at most 1024"""
+    return min(8 * L, 1024)
 METRIC_L2, METRIC_MIPS = 0, 1
 GT_PLAIN, GT_MFMA, GT_REFTIES, GT_MIPS = 0, 1, 0x100, 0x200
 SENT = 0xFFFFFFFF
@@ -238,22 +243,26 @@ def l2_batch(index, q_dev, ids_dev):
 
 def search(index, q_dev, L, k, mode=SEARCH_REF, cut=1.35, limit=None, degree_limit=None,
            entry=0, start_points=None, fast=None, stats=True, out=None, frontier=False,
-           edges=False):
+           edges=False, visited=False):
     """Beam search for every row of q_dev (uint8 tensor, nq x stride bytes).
     frontier=True returns each query's whole frontier (nq x L) instead of
     its first k (the reference mode still cuts with k).  edges=True (FAST
-    mode): stats rows are {visited, dist_cmps, adjacency entries read}."""
+    mode): stats rows are {visited, dist_cmps, adjacency entries read}.
+    visited=True (FAST mode): ids are each query's expanded nodes in
+    expansion order (nq x visited_cap(L), 0xFFFFFFFF past the count)."""
     torch = _torch()
     nq = q_dev.shape[0]
     qstride = q_dev.shape[1] * q_dev.element_size()
     qp = QueryParams(k, cut, L, index.n if limit is None else limit,
                      index.R if degree_limit is None else degree_limit)
     sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
-    width = L if frontier else k
+    width = visited_cap(L) if visited else (L if frontier else k)
     if frontier:
         mode |= SEARCH_FRONTIER
     if edges:
         mode |= SEARCH_EDGES
+    if visited:
+        mode |= SEARCH_VISITED
     if out is None:
         ids = torch.empty((nq, width), dtype=torch.int32, device="cuda")
         d = torch.empty((nq, width), dtype=torch.float32, device="cuda")

@@ -399,7 +399,8 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                 uint32_t entry, const uint32_t *seeds, uint32_t nseeds, uint32_t L, uint32_t k,
                 const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
                 uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
-                uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w)
+                uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w,
+                uint32_t *vis_out, uint32_t vis_cap, bool vis_cmps)
 {
     nbpg_fast_config c = {2, 1, 11, 0, 0};
     if (cfg)
@@ -463,6 +464,11 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     a.out_dists = d_dists;
     a.out_stats = d_stats;
     a.stats_w = stats_w;
+    a.vis_out = vis_out;
+    a.vis_cap = vis_out ? vis_cap : 0;
+    a.vis_cmps = vis_out && vis_cmps ? 1u : 0u;
+    if (vis_out && (!reg || vis_cap == 0))
+        return set_error(NBPG_EUNSUP, "visited lists need L <= 256");
     const uint64_t lds = (uint64_t)a.wave_bytes * c.waves_per_group;
     if (lds > (uint64_t)g_dev.lds_max)
         return set_error(NBPG_EUNSUP, "fast search needs %llu B of LDS per group",
@@ -573,13 +579,24 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
 {
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
     const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
-    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES);
-    if (edges && base != NBPG_SEARCH_FAST)
-        return set_error(NBPG_EINVAL, "NBPG_SEARCH_EDGES counts are kept by the FAST kernels");
+    const bool visited = (mode & NBPG_SEARCH_VISITED) != 0;
+    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES | NBPG_SEARCH_VISITED);
+    if ((edges || visited) && base != NBPG_SEARCH_FAST)
+        return set_error(NBPG_EINVAL, "NBPG_SEARCH_EDGES / NBPG_SEARCH_VISITED are FAST-mode outputs");
+    if (visited && frontier)
+        return set_error(NBPG_EINVAL, "NBPG_SEARCH_VISITED replaces the frontier output");
     NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, base,
-                          frontier || qp->k, d_out_ids));
+                          frontier || visited || qp->k, d_out_ids));
     if (nq == 0)
         return NBPG_OK;
+    if (visited) {
+        if (qp->L > 4 * WAVE)
+            return set_error(NBPG_EUNSUP, "visited lists need L <= 256");
+        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
+                           sp_num, qp->L, qp->k, nullptr, nullptr, qp->L, qp->k, qp->limit, fast,
+                           nullptr, nullptr, d_out_stats, counter ? counter : work_counter(ix),
+                           as_stream(stream), edges ? 3 : 2, d_out_ids, NBPG_VISITED_CAP(qp->L));
+    }
     const uint32_t out_k = frontier ? qp->L : qp->k;
     if (base == NBPG_SEARCH_REF)
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, out_k,
@@ -619,8 +636,8 @@ static int search_rerank_impl(const nbpg_index *ix, const void *d_queries, uint6
                          (unsigned long long)raw_query_stride);
     if (nq && (!d_raw_base || !d_raw_queries))
         return set_error(NBPG_EINVAL, "null raw rows");
-    if (mode == NBPG_SEARCH_FAST && rerank_factor == 0)
-        return set_error(NBPG_EUNSUP, "filtering mode (rerank_factor 0) needs NBPG_SEARCH_REF");
+    if (mode == NBPG_SEARCH_FAST && rerank_factor == 0 && qp->L > 4 * WAVE)
+        return set_error(NBPG_EUNSUP, "FAST filtering mode needs L <= 256");
     if (nq == 0 || qp->k == 0)
         return NBPG_OK;
     hipStream_t s = as_stream(stream);
@@ -630,15 +647,19 @@ static int search_rerank_impl(const nbpg_index *ix, const void *d_queries, uint6
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, qp->k, &ex,
                           d_out_ids, d_out_dists, d_out_stats, s, fixed, fixed_bytes);
     }
-    // FAST: the top L of each beam into scratch, then re-scored
-    if (std::min<uint64_t>((uint64_t)qp->k * rerank_factor, qp->L) > RERANK_MAX)
+    // FAST: the top L of each beam (rerank) or its visited list (filtering)
+    // into scratch, then re-scored
+    const bool filtering = rerank_factor == 0;
+    const uint32_t width = filtering ? NBPG_VISITED_CAP(qp->L) : qp->L;
+    if (!filtering && std::min<uint64_t>((uint64_t)qp->k * rerank_factor, qp->L) > RERANK_MAX)
         return set_error(NBPG_EUNSUP, "min(k * rerank_factor, L) > %d", RERANK_MAX);
-    const uint64_t cand_bytes = nq * qp->L * sizeof(uint32_t);
+    const uint64_t cand_bytes = nq * width * sizeof(uint32_t);
     uint32_t *cand;
     if (fixed) {
         if (cand_bytes > fixed_bytes)
-            return set_error(NBPG_EINVAL, "nq * L = %llu candidates exceed the %llu-byte scratch",
-                             (unsigned long long)(nq * qp->L), (unsigned long long)fixed_bytes);
+            return set_error(NBPG_EINVAL, "%llu candidates (nq x %u) exceed the %llu-byte scratch",
+                             (unsigned long long)(nq * width), width,
+                             (unsigned long long)fixed_bytes);
         cand = reinterpret_cast<uint32_t *>(fixed);
     } else {
         NBPG_TRY(ensure_ws(const_cast<nbpg_index *>(ix), cand_bytes));
@@ -646,6 +667,16 @@ static int search_rerank_impl(const nbpg_index *ix, const void *d_queries, uint6
     }
     // every query writes all L of its slots (NBPG_SENT past its frontier),
     // so the scratch needs no clearing between calls
+    if (filtering) {
+        // every visited entry is a candidate: k * factor >= the list width
+        NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
+                             sp_num, qp->L, qp->k, nullptr, nullptr, qp->L, qp->k, qp->limit,
+                             fast, nullptr, nullptr, d_out_stats, work_counter(ix), s, 2, cand,
+                             width, true));
+        return rerank_launch(raw_dtype, ix->mips, d_raw_base, ix->n, raw_dims, d_raw_queries,
+                             raw_query_stride, nq, cand, width, width, qp->k,
+                             (width + qp->k - 1) / qp->k, d_out_ids, d_out_dists, s);
+    }
     NBPG_TRY(search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
                          sp_num, qp->L, qp->L, nullptr, nullptr, qp->L, qp->L, qp->limit, fast,
                          cand, nullptr, d_out_stats, work_counter(ix), s));

@@ -37,6 +37,11 @@ struct SearchArgs {
     float *out_dists;
     uint32_t *out_stats;
     uint32_t stats_w; // u32 per stats row: 2 {visited, dist_cmps} or 3 (+ adjacency entries read)
+    // FAST: the expanded nodes in expansion order, vis_cap per query, ids
+    // past the count NBPG_SENT (NULL: not kept)
+    uint32_t *vis_out;
+    uint32_t vis_cap;
+    uint32_t vis_cmps; // 1: the list is re-scored afterwards, count it in dist_cmps
 };
 
 #define FAST_FLAG 0x80000000ull // bit 31 of the id word: entry expanded
@@ -85,11 +90,14 @@ __device__ __forceinline__ uint32_t fast_slot(uint32_t id, uint32_t flog2)
 // FAST-mode launch (search.hip); counter is the launch's own work counter
 // seeds: nseeds > 1 distinct start nodes (host array) instead of entry
 // stats_w: 2 or 3 u32 per stats row (3 adds the adjacency entries read)
+// vis_out: also the first vis_cap expanded nodes of each query (d_ids may
+// then be NULL); L <= 256 only
 int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
                 uint32_t entry, const uint32_t *seeds, uint32_t nseeds, uint32_t L, uint32_t k,
                 const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
                 uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
-                uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w = 2);
+                uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w = 2,
+                uint32_t *vis_out = nullptr, uint32_t vis_cap = 0, bool vis_cmps = false);
 
 int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
                       dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a);

@@ -178,6 +178,9 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             }
             if (got == 0)
                 break;
+            if (a.vis_out && lane < got && nexp + lane < a.vis_cap)
+                a.vis_out[(uint64_t)qi * a.vis_cap + nexp + lane] =
+                    lane == 0 ? s0 : (lane == 1 ? s1 : (lane == 2 ? s2 : s3));
             nexp += got;
             // gather adjacency rows, filter
             uint32_t ids[IPL];
@@ -237,10 +240,20 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
             }
             wave_sync();
         }
+        if (a.vis_out) {
+            // the rest of the list ends it; when the list is re-scored
+            // (filtering mode, beamSearch.h:239-246) that counts as distance
+            // computations
+            const uint32_t nv = min(nexp, a.vis_cap);
+            for (uint32_t i = nv + lane; i < a.vis_cap; i += WAVE)
+                a.vis_out[(uint64_t)qi * a.vis_cap + i] = NBPG_SENT;
+            if (a.vis_cmps)
+                cmps += nv;
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t gp = lane + WAVE * e;
-            if (gp < a.out_k) {
+            if (a.out_ids && gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
                 const uint64_t v = unflag(b[e]);
@@ -434,6 +447,9 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
             }
             if (got == 0)
                 break;
+            if (a.vis_out && lane < got && nexp + lane < a.vis_cap)
+                a.vis_out[(uint64_t)qi * a.vis_cap + nexp + lane] =
+                    lane == 0 ? s0 : (lane == 1 ? s1 : (lane == 2 ? s2 : s3));
             nexp += got;
             uint32_t ids[IPL];
 #pragma unroll
@@ -468,10 +484,20 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
             score_list(nf);
             wave_sync();
         }
+        if (a.vis_out) {
+            // the rest of the list ends it; when the list is re-scored
+            // (filtering mode, beamSearch.h:239-246) that counts as distance
+            // computations
+            const uint32_t nv = min(nexp, a.vis_cap);
+            for (uint32_t i = nv + lane; i < a.vis_cap; i += WAVE)
+                a.vis_out[(uint64_t)qi * a.vis_cap + i] = NBPG_SENT;
+            if (a.vis_cmps)
+                cmps += nv;
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t gp = lane + WAVE * e;
-            if (gp < a.out_k) {
+            if (a.out_ids && gp < a.out_k) {
                 const uint64_t o = (uint64_t)qi * a.out_k + gp;
                 const bool ok = gp < k && b[e] != EMPTY;
                 const uint64_t v = unflag(b[e]);

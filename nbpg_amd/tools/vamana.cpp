@@ -19,8 +19,9 @@
 // rows of another size than the quantized ones) the sweep searches the
 // quantized rows and takes final distances from the raw ones
 // (beam_search_rerank, beamSearch.h:183-247): --rerank_factor > 0 re-scores
-// the frontier head, 0 is the filtering mode (reference mode only); the raw
-// rows stay in device memory (--cache_raw is accepted and changes nothing).
+// the frontier head, 0 is the filtering mode (re-scores the visited list;
+// with --search_mode fast for beams up to 256); the raw rows stay in device
+// memory (--cache_raw is accepted and changes nothing).
 // 4-split builds draw their random start points from --sp_seed instead of
 // std::random_device.  --use_mips builds and searches with the negated
 // inner product (DistanceMetric::MIPS).  Not on this path (refused):

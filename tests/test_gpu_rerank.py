@@ -152,6 +152,52 @@ def test_frontier_output_and_fast_rerank(gpu, golden_exact, name, cdt, xdt):
         f, fd, fst = N.search_rerank(ix, qd, xb, xq, L, k, rf, mode=N.SEARCH_FAST)
         assert (c == f).all() and (cd == fd).all(), (L, k, rf)
         assert (fst[:, 0] > 0).all()
+    ix.close()
+
+
+@pytest.mark.parametrize("name,cdt,xdt", WORKLOADS)
+def test_fast_filtering_and_visited_lists(gpu, golden_exact, name, cdt, xdt):
+    """FAST filtering mode (rerank_factor 0, beamSearch.h:239-246): each
+    query's visited list (NBPG_SEARCH_VISITED: distinct ids, as many as the
+    search's visited count, holding the whole final beam) re-scored with the
+    raw rows equals the stand-alone rerank of that list, and dist_cmps grows
+    by the list's length.  Since the list holds the beam, the i-th exact
+    distance of the filtering result is never above the i-th of the beam
+    re-scored."""
+    import torch
+    N, z = gpu, golden_exact
+    ix, qd, xb, xq = _exact_index(N, z, name, cdt)
+    for L, k in [(16, 10), (20, 10), (40, 5), (128, 100), (256, 10)]:
+        cap = N.visited_cap(L)
+        vis, _, vst = N.search(ix, qd, L, k, mode=N.SEARCH_FAST, visited=True)
+        v = vis.cpu().numpy().view(np.uint32)
+        vst = vst.cpu().numpy()
+        assert v.shape == (len(qd), cap)
+        live = v != 0xFFFFFFFF
+        cnt = live.sum(1)
+        assert (cnt == np.minimum(vst[:, 0], cap)).all(), (L, k)
+        assert all(live[i, :cnt[i]].all() for i in range(len(v)))  # contiguous
+        assert all(len(set(v[i, :cnt[i]])) == cnt[i] for i in range(len(v)))
+        beam, _, bst = N.search(ix, qd, L, L, mode=N.SEARCH_FAST)
+        b = beam.cpu().numpy().view(np.uint32)
+        bst = bst.cpu().numpy()
+        assert (bst == vst).all(), (L, k)
+        for i in range(len(v)):
+            bi = b[i][b[i] != 0xFFFFFFFF]
+            if cnt[i] < cap:
+                assert set(bi) <= set(v[i, :cnt[i]]), (L, k, i)
+        f, fd, fst = N.search_rerank(ix, qd, xb, xq, L, k, 0, mode=N.SEARCH_FAST)
+        a, ad = N.rerank(xb, xq, vis, k, (cap + k - 1) // k)
+        assert torch.equal(f, a) and torch.equal(fd, ad), (L, k)
+        fst = fst.cpu().numpy()
+        assert (fst[:, 0] == vst[:, 0]).all()
+        assert (fst[:, 1] == vst[:, 1] + np.minimum(vst[:, 0], cap)).all(), (L, k)
+        c, cd = N.rerank(xb, xq, beam, k, (L + k - 1) // k)
+        whole = cnt < cap  # lists that were not cut at the capacity
+        assert whole.any()
+        assert (fd.cpu().numpy()[whole] <= cd.cpu().numpy()[whole]).all(), (L, k)
     with pytest.raises(N.NbpgError):
-        N.search_rerank(ix, qd, xb, xq, 20, 10, 0, mode=N.SEARCH_FAST)
+        N.search(ix, qd, 300, 10, mode=N.SEARCH_FAST, visited=True)
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 20, 10, mode=N.SEARCH_REF, visited=True)
     ix.close()

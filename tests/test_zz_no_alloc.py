@@ -1,7 +1,7 @@
 """nbpg_search_rerank_noalloc (scratch in the library's fixed per-GPU array,
 no device allocation) returns exactly what nbpg_search_rerank returns: ids,
 distances and {visited, dist_cmps} counters, bit for bit, in FAST mode (nq x
-L candidate ids in the scratch) and in reference mode at an L whose search
+L candidate ids, or nq visited lists, in the scratch) and in reference mode at an L whose search
 state does not fit in LDS (per-workgroup state in the scratch, a smaller
 grid).  Each case runs two calls in a row, the second with fewer queries,
 so the second call reads a scratch that still holds the first call's
@@ -42,6 +42,7 @@ def _same(a, b):
 @pytest.mark.parametrize("mode,L,k,rf,sizes", [
     ("fast", 64, 10, 4, (400, 150)),
     ("fast", 200, 10, 100, (300, 37)),
+    ("fast", 32, 10, 0, (400, 90)),    # filtering: visited lists in the scratch
     ("ref", 256, 10, 2, (200, 60)),    # state beyond LDS: global scratch, smaller grid
     ("ref", 256, 10, 0, (120, 31)),    # filtering mode
     ("ref", 48, 10, 3, (200, 90)),     # state in LDS

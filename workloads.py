@@ -388,6 +388,9 @@ def run_refgraph(args, log):
     gi, gd = N.compgt(bd, torch.from_numpy(q).cuda(), 100, N.U8, N.GT_MFMA)
     gt, gtd = gi.cpu().numpy().view(np.uint32), gd.cpu().numpy()
     qd = N.to_device(pad_rows(q, 128))
+    # timing batches: the query set and three fresh draws of the same mixture
+    qds = [qd] + [N.to_device(pad_rows(D.workload_1m_queries(nq, b), 128)) for b in (1, 2, 3)]
+    from nbpg_amd import autotune as AT
     res = {"workload": "refgraph", "n": n, "nq": nq, "reference_build_s": round(ref_build_s, 1),
            "reference_binary": os.path.basename(exe), "graphs": []}
     for name in ("reference", "device"):
@@ -416,8 +419,18 @@ def run_refgraph(args, log):
             reps = 10 if mode == N.SEARCH_FAST else 3
             ms = events_ms(lambda: N.search(ix, qd, L, 10, mode=mode, start_points=sp,
                                             fast=(2, 1, 10, 0, 0), out=out), reps)
-            row[tag] = {"L": L, "recall@10": round(r, 4), "qps": round(nq / ms * 1e3, 1)}
+            # consecutive batches (the four query sets in turn) on 4 streams
+            M = AT.output_sets(len(qds), 4)
+            outs = [N.search(ix, qds[j % 4], L, 10, mode=mode, start_points=sp,
+                             fast=(2, 1, 10, 0, 0)) for j in range(M)]
+            ms4 = AT.pipelined_ms(lambda i: N.search(ix, qds[i % 4], L, 10, mode=mode,
+                                                     start_points=sp, fast=(2, 1, 10, 0, 0),
+                                                     out=outs[i % M]), 4, 24 if reps == 10 else 8)
+            row[tag] = {"L": L, "recall@10": round(r, 4), "qps": round(nq / ms * 1e3, 1),
+                        "qps_4_streams": round(nq / ms4 * 1e3, 1)}
         row["fast_over_plain"] = round(row["fast"]["qps"] / row["plain"]["qps"], 2)
+        row["fast_over_plain_4_streams"] = round(row["fast"]["qps_4_streams"]
+                                                 / row["plain"]["qps_4_streams"], 2)
         log("refgraph", row)
         res["graphs"].append(row)
         ix.close()
