@@ -119,6 +119,7 @@ void mf_scan_kernel(MfArgs a)
     constexpr int32_t MISSING = 1 << 30;
     __shared__ __align__(16) uint8_t rows_lds[2][MF_BLK * PITCH];
     __shared__ __align__(16) int32_t w_lds[2][MF_BLK];
+    __shared__ __align__(16) int32_t i_lds[2][MF_BLK];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint32_t col = lane % 16, kq = lane / 16;
     const uint64_t q0 = ((uint64_t)blockIdx.x * MF_WAVES + wv) * (16 * MF_QW);
@@ -161,30 +162,36 @@ void mf_scan_kernel(MfArgs a)
     constexpr uint32_t PIECES = (MF_BLK * 4 * KS + 64 * MF_WAVES - 1) / (64 * MF_WAVES);
     v4i stage[2][PIECES];
     int32_t wstage[2] = {MISSING, MISSING};
+    const uint64_t row_bytes = a.row_step * a.dims;
+    const uint32_t row_bytes32 = (uint32_t)row_bytes;
     // register set R (a compile-time index: register arrays take no
-    // run-time index) <- block b0
+    // run-time index) <- block b0; the block's base address is uniform, each
+    // piece adds its row and chunk offset
     auto fetch = [&](uint64_t b0, auto R) {
+        const uint8_t *bb = a.base + b0 * row_bytes;
+        const uint64_t left = b0 < r1 ? r1 - b0 : 0; // rows of this block that exist
 #pragma unroll
         for (uint32_t j = 0; j < PIECES; ++j) {
             const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
-            const uint64_t row = b0 + p / (4 * KS);
-            const uint32_t off = 16 * (p % (4 * KS));
+            const uint32_t prow = p / (4 * KS), off = 16 * (p % (4 * KS));
             v4i v = {0, 0, 0, 0};
-            if (p < MF_BLK * 4 * KS && row < r1 && off < a.dims) {
-                const uint4 u = *reinterpret_cast<const uint4 *>(a.base + row * a.row_step * a.dims +
-                                                                 off);
+            if (p < MF_BLK * 4 * KS && off < a.dims && prow < left) {
+                // a block spans under 2^31 bytes (the host bounds the
+                // sample stride), so the offset fits 32 bits
+                const uint4 u = *reinterpret_cast<const uint4 *>(bb + (prow * row_bytes32 + off));
                 v = v4i{(int)(u.x ^ xmask), (int)(u.y ^ xmask), (int)(u.z ^ xmask),
                         (int)(u.w ^ xmask)};
             }
             stage[R.value][j] = v;
         }
         if (threadIdx.x < MF_BLK) {
-            const uint64_t row = b0 + threadIdx.x;
-            const int32_t x = row < r1 ? (int32_t)a.nx[row * a.row_step] : 0;
-            wstage[R.value] = row < r1 ? (MIPS ? -x : x) : MISSING;
+            const bool ok = threadIdx.x < left;
+            const int32_t x = ok ? (int32_t)a.nx[b0 * a.row_step + threadIdx.x * a.row_step] : 0;
+            wstage[R.value] = ok ? (MIPS ? -x : x) : MISSING;
         }
     };
-    // LDS buffer buf <- register set R
+    // LDS buffer buf <- register set R; each row's w and the accumulator's
+    // start value -floor(w / 2) (L2) or -w (inner product)
     auto put = [&](int buf, auto R) {
 #pragma unroll
         for (uint32_t j = 0; j < PIECES; ++j) {
@@ -193,8 +200,11 @@ void mf_scan_kernel(MfArgs a)
                 *reinterpret_cast<v4i *>(&rows_lds[buf][(p / (4 * KS)) * PITCH + 16 * (p % (4 * KS))]) =
                     stage[R.value][j];
         }
-        if (threadIdx.x < MF_BLK)
-            w_lds[buf][threadIdx.x] = wstage[R.value];
+        if (threadIdx.x < MF_BLK) {
+            const int32_t w = wstage[R.value];
+            w_lds[buf][threadIdx.x] = w;
+            i_lds[buf][threadIdx.x] = THRESH ? 0 : (MIPS ? -w : -(w >> 1));
+        }
     };
     // every query tile against the block in LDS buffer buf (rows b0..)
     auto score = [&](int buf, uint64_t b0) {
@@ -207,18 +217,13 @@ void mf_scan_kernel(MfArgs a)
             for (int s = 0; s < KS; ++s)
                 b[s] = *reinterpret_cast<const v4i *>(
                     &rows_lds[buf][(16 * blk + col) * PITCH + 64 * s + 16 * kq]);
-            const v4i w4 = *reinterpret_cast<const v4i *>(&w_lds[buf][16 * blk + 4 * kq]);
             // scan: the accumulator starts at -h (h = floor(w / 2), L2) or
             // -w (inner product), shared by every query tile, so the matrix
             // core returns c' = c - h or c - w.  e = w - 2c <= U implies
             // c' >= -floor(U / 2) (a superset that adds only e = U + 1), and
             // e = w - c <= U is c' >= -U: one compare on the largest of the
             // four values per tile
-            v4i init = {0, 0, 0, 0};
-            if (!THRESH)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    init[r] = MIPS ? -w4[r] : -(w4[r] >> 1);
+            const v4i init = *reinterpret_cast<const v4i *>(&i_lds[buf][16 * blk + 4 * kq]);
             // chunk-major order: each accumulator's next product is issued
             // MF_QW products after its last, so none waits on its own
             // result; one wave-wide branch per sub-block
@@ -232,6 +237,7 @@ void mf_scan_kernel(MfArgs a)
                 for (int t = 0; t < MF_QW; ++t)
                     c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b[s], qf[t][s], c[t], 0, 0, 0);
             if (THRESH) {
+                const v4i w4 = *reinterpret_cast<const v4i *>(&w_lds[buf][16 * blk + 4 * kq]);
 #pragma unroll
                 for (int t = 0; t < MF_QW; ++t)
 #pragma unroll
@@ -248,19 +254,24 @@ void mf_scan_kernel(MfArgs a)
                 if (ballot(hit)) {
 #pragma unroll
                     for (int t = 0; t < MF_QW; ++t) {
-                        const uint64_t seg = ((q0 + 16 * t + col) * gridDim.y + blockIdx.y) * 4 + kq;
+                        // only the lanes of tiles with a kept row
+                        if (max(max(c[t][0], c[t][1]), max(c[t][2], c[t][3])) >= V[t]) {
+                            const uint64_t seg =
+                                ((q0 + 16 * t + col) * gridDim.y + blockIdx.y) * 4 + kq;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (c[t][r] >= V[t]) {
-                                // the exact partial distance, from c' and the row's w
-                                const int32_t e =
-                                    MIPS ? -c[t][r] : w4[r] - 2 * (c[t][r] + (w4[r] >> 1));
-                                const uint64_t row = b0 + 16 * blk + 4 * kq + r;
-                                if (kept[t] < a.seg_cap)
-                                    a.cand[seg * a.seg_cap + kept[t]] =
-                                        make_key((float)e, (uint32_t)row);
-                                ++kept[t];
-                            }
+                            for (int r = 0; r < 4; ++r)
+                                if (c[t][r] >= V[t]) {
+                                    // the exact partial distance from c' and w:
+                                    // e = w - 2c = (w & 1) - 2c' (L2), -c' (IP)
+                                    const int32_t w = w_lds[buf][16 * blk + 4 * kq + r];
+                                    const int32_t e = MIPS ? -c[t][r] : (w & 1) - 2 * c[t][r];
+                                    const uint64_t row = b0 + 16 * blk + 4 * kq + r;
+                                    if (kept[t] < a.seg_cap)
+                                        a.cand[seg * a.seg_cap + kept[t]] =
+                                            make_key((float)e, (uint32_t)row);
+                                    ++kept[t];
+                                }
+                        }
                     }
                 }
             }
@@ -361,7 +372,9 @@ __global__ __launch_bounds__(256) void mf_bound_kernel(const int32_t *__restrict
     }
 }
 
-#define MF_SEL_MAX 4096 // rows one select workgroup sorts
+// rows one select workgroup sorts: 16 KB of LDS, so a CU holds 8 of them;
+// the bound keeps about 4k rows per query, a query with more is redone
+#define MF_SEL_MAX 2048
 #define MF_SEL_SEGS 2048
 
 // per query: gather its nseg segments, sort the kept rows by (partial
@@ -570,8 +583,10 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     const uint32_t KS = (dims + 63) / 64;
     const bool im = mips != 0;
     // sample stride: the bound's sample rank is about J, so about J * step
-    // (4k) base rows are kept
-    const uint64_t step = std::max<uint64_t>(1, std::min<uint64_t>(4ull * k / MF_J, n / 64));
+    // (4k) base rows are kept; a staged block of sample rows spans under
+    // 2^31 bytes (the scan's 32-bit piece offsets)
+    const uint64_t step = std::max<uint64_t>(
+        1, std::min<uint64_t>({4ull * k / MF_J, n / 64, ((1ull << 31) - 1) / (MF_BLK * (uint64_t)dims)}));
     const uint64_t S = (n + step - 1) / step;
     // equal chunks of at most MF_QCHUNK queries
     const uint64_t nchunks = (nq + MF_QCHUNK - 1) / MF_QCHUNK;
