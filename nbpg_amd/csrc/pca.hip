@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void pca_bf16_kernel(PcaArgs a)
 // and each of its wavefronts then projects one row block reading W
 // fragments from LDS (one copy of W per PCA_LDS_WAVES blocks instead of one
 // read per block).
-#define PCA_LDS_WAVES 12
+#define PCA_LDS_WAVES 16
 template <int XT, int NT>
 __global__ __launch_bounds__(64 * PCA_LDS_WAVES) void pca_bf16_lds_kernel(PcaArgs a)
 {
