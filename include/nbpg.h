@@ -267,12 +267,13 @@ int nbpg_search_rerank_noalloc(const nbpg_index *idx, const void *d_queries, uin
 /* FAST mode with per-query beam width and k (uneven batches); d_L, d_k are
  * nq u32 each on the device, outputs have stride out_k per query.  A query's
  * L is capped at max_L (and raised to 1), its k at min(L, out_k), by every
- * kernel shape alike. */
+ * kernel shape alike.  Start points as nbpg_search's FAST mode (1..64
+ * distinct host ids; more than one needs max_L <= 256). */
 int nbpg_search_uneven(const nbpg_index *idx, const void *d_queries, uint64_t nq,
                        uint64_t query_stride, const uint32_t *d_L, const uint32_t *d_k,
-                       uint32_t max_L, uint32_t out_k, uint32_t entry,
-                       const nbpg_fast_config *fast, uint32_t *d_out_ids, float *d_out_dists,
-                       uint32_t *d_out_stats, void *stream);
+                       uint32_t max_L, uint32_t out_k, const uint32_t *h_start_points,
+                       uint32_t sp_num, const nbpg_fast_config *fast, uint32_t *d_out_ids,
+                       float *d_out_dists, uint32_t *d_out_stats, void *stream);
 
 /* ---- RobustPrune: Prune::sort_candidates + Prune::robustPrune
  * (prune.h:49-87, 158-209) for a batch of nodes.  Node t's candidates are
@@ -345,8 +346,9 @@ typedef struct {
     uint32_t max_batch_queries; /* queries per batch */
     uint64_t mem_budget;        /* device bytes for all slots' buffers (0 = unbounded) */
     uint32_t max_L, max_k;      /* largest L and k a request may ask for */
-    uint32_t entry;             /* entry node */
     nbpg_fast_config fast;      /* launch shape of every batch */
+    uint32_t sp_num;            /* start points: 1..64 distinct ids (more than one: max_L <= 256) */
+    uint32_t start_points[64];  /* e.g. from nbpg_pick_seeds */
 } nbpg_server_config;
 
 typedef struct {

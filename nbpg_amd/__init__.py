@@ -63,7 +63,8 @@ U32, U64, I32, DBL = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 
 class ServerConfig(C.Structure):
     _fields_ = [("slots", U32), ("max_batch_queries", U32), ("mem_budget", U64),
-                ("max_L", U32), ("max_k", U32), ("entry", U32), ("fast", FastConfig)]
+                ("max_L", U32), ("max_k", U32), ("fast", FastConfig), ("sp_num", U32),
+                ("start_points", U32 * 64)]
 
 
 class ServerStats(C.Structure):
@@ -99,8 +100,8 @@ _SIGS = {
     "nbpg_search_graph_destroy": (I32, [VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
                           C.POINTER(FastConfig), VP, VP, VP, VP]),
-    "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, U32, C.POINTER(FastConfig),
-                                 VP, VP, VP, VP]),
+    "nbpg_search_uneven": (I32, [VP, VP, U64, U64, VP, VP, U32, U32, VP, U32,
+                                 C.POINTER(FastConfig), VP, VP, VP, VP]),
     "nbpg_robust_prune_batch": (I32, [VP, VP, U64, VP, VP, U32, VP, DBL, DBL, VP, VP, VP, VP]),
     "nbpg_compgt": (I32, [I32, VP, U64, VP, U64, U32, U32, I32, VP, VP, VP]),
     "nbpg_build": (I32, [VP, C.POINTER(BuildParams), C.POINTER(U64)]),
@@ -352,8 +353,12 @@ def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100
     return ids, d, st
 
 
-def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None, out=None):
+def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None, out=None,
+                  start_points=None):
+    """FAST search with per-query L and k (device u32 arrays); start points:
+    start_points (up to 64 ids) or the single entry."""
     torch = _torch()
+    sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
     nq = q_dev.shape[0]
     qstride = q_dev.shape[1] * q_dev.element_size()
     if out is None:
@@ -364,7 +369,8 @@ def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None, 
         ids, d, st = out
     fc = FastConfig(*fast) if fast is not None else None
     _check(lib().nbpg_search_uneven(index.h, _ptr(q_dev), nq, qstride, _ptr(L_dev), _ptr(k_dev),
-                                    max_L, out_k, entry, C.byref(fc) if fc is not None else None,
+                                    max_L, out_k, sp.ctypes.data, len(sp),
+                                    C.byref(fc) if fc is not None else None,
                                     _ptr(ids), _ptr(d), _ptr(st), _stream()), "nbpg_search_uneven")
     return ids, d, st
 
@@ -375,11 +381,14 @@ class Server:
     wait() releases the GIL while it blocks."""
 
     def __init__(self, index, slots=3, max_batch_queries=4096, mem_budget=0, max_L=256,
-                 max_k=100, entry=0, fast=(2, 1, 11, 0, 0)):
+                 max_k=100, entry=0, fast=(2, 1, 11, 0, 0), start_points=None):
         _torch()
         self.index = index
-        cfg = ServerConfig(slots, max_batch_queries, mem_budget, max_L, max_k, entry,
-                           FastConfig(*fast))
+        sp = [entry] if start_points is None else [int(x) for x in start_points]
+        if not 1 <= len(sp) <= 64:
+            raise ValueError("1..64 start points")
+        cfg = ServerConfig(slots, max_batch_queries, mem_budget, max_L, max_k, FastConfig(*fast),
+                           len(sp), (U32 * 64)(*sp))
         h = VP()
         _check(lib().nbpg_server_create(C.byref(h), index.h, C.byref(cfg)), "nbpg_server_create")
         self.h = h

@@ -717,24 +717,27 @@ int nbpg_search_rerank_noalloc(const nbpg_index *ix, const void *d_queries, uint
 
 int nbpg_search_uneven(const nbpg_index *ix, const void *d_queries, uint64_t nq,
                        uint64_t query_stride, const uint32_t *d_L, const uint32_t *d_k,
-                       uint32_t max_L, uint32_t out_k, uint32_t entry, const nbpg_fast_config *fast,
-                       uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats, void *stream)
+                       uint32_t max_L, uint32_t out_k, const uint32_t *h_start_points,
+                       uint32_t sp_num, const nbpg_fast_config *fast, uint32_t *d_out_ids,
+                       float *d_out_dists, uint32_t *d_out_stats, void *stream)
 {
-    if (!ix || (nq && (!d_queries || !d_out_ids || !d_L || !d_k)))
+    if (!ix || !h_start_points || (nq && (!d_queries || !d_out_ids || !d_L || !d_k)))
         return set_error(NBPG_EINVAL, "null argument");
     if (max_L == 0 || out_k == 0 || out_k > max_L)
         return set_error(NBPG_EINVAL, "max_L %u / out_k %u", max_L, out_k);
-    if (entry >= ix->n)
-        return set_error(NBPG_EINVAL, "entry %u >= n", entry);
+    if (sp_num == 0 || sp_num > 64)
+        return set_error(NBPG_EINVAL, "start points: %u (1..64)", sp_num);
+    if (h_start_points[0] >= ix->n)
+        return set_error(NBPG_EINVAL, "start point %u >= n", h_start_points[0]);
     if (query_stride < ix->dims)
         return set_error(NBPG_EINVAL, "query stride too small");
     if (nq == 0)
         return NBPG_OK;
     if (nq >= 0xFFFFFFFFull)
         return set_error(NBPG_EUNSUP, "nq too large");
-    return search_fast(ix, d_queries, nq, query_stride, entry, nullptr, 0, 0, 0, d_L, d_k, max_L,
-                       out_k, ix->n, fast, d_out_ids, d_out_dists, d_out_stats, work_counter(ix),
-                       as_stream(stream));
+    return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points, sp_num,
+                       0, 0, d_L, d_k, max_L, out_k, ix->n, fast, d_out_ids, d_out_dists,
+                       d_out_stats, work_counter(ix), as_stream(stream));
 }
 
 } // extern "C"

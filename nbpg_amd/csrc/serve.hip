@@ -144,9 +144,9 @@ static int launch_slot(nbpg_server *sv, Slot &s)
     HIP_TRY(hipMemcpyAsync(s.dq, s.hq, (size_t)off * sv->stride, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dL, s.hL, off * 4ull, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dk, s.hk, off * 4ull, hipMemcpyHostToDevice, s.stream));
-    NBPG_TRY(search_fast(sv->ix, s.dq, off, sv->stride, sv->cfg.entry, nullptr, 0, 0, 0, s.dL,
-                         s.dk, Lmax, kmax, sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr,
-                         s.counter, s.stream));
+    NBPG_TRY(search_fast(sv->ix, s.dq, off, sv->stride, sv->cfg.start_points[0],
+                         sv->cfg.start_points, sv->cfg.sp_num, 0, 0, s.dL, s.dk, Lmax, kmax,
+                         sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr, s.counter, s.stream));
     HIP_TRY(hipMemcpyAsync(s.hids, s.dids, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipMemcpyAsync(s.hd, s.dd, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipEventRecord(s.ev, s.stream));
@@ -243,8 +243,17 @@ int nbpg_server_create(nbpg_server **out, const nbpg_index *ix, const nbpg_serve
     if (!out || !ix || !cfg)
         return set_error(NBPG_EINVAL, "null argument");
     if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch_queries == 0 || cfg->max_L == 0 ||
-        cfg->max_k == 0 || cfg->max_k > cfg->max_L || cfg->entry >= ix->n)
+        cfg->max_k == 0 || cfg->max_k > cfg->max_L || cfg->sp_num == 0 || cfg->sp_num > 64)
         return set_error(NBPG_EINVAL, "server config out of range");
+    for (uint32_t i = 0; i < cfg->sp_num; ++i) {
+        if (cfg->start_points[i] >= ix->n)
+            return set_error(NBPG_EINVAL, "start point %u >= n", cfg->start_points[i]);
+        for (uint32_t j = 0; j < i; ++j)
+            if (cfg->start_points[j] == cfg->start_points[i])
+                return set_error(NBPG_EINVAL, "start point %u repeated", cfg->start_points[i]);
+    }
+    if (cfg->sp_num > 1 && cfg->max_L > 4 * WAVE)
+        return set_error(NBPG_EUNSUP, "several start points need max_L <= 256");
     nbpg_server *sv = new nbpg_server();
     sv->ix = ix;
     sv->cfg = *cfg;

@@ -189,3 +189,43 @@ def test_uneven_L_capped_at_max_L(served):
         b = N.search_uneven(ix, qd, eq, ks, max_L, 10, fast=FAST)
         assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), max_L
         assert torch.equal(a[2], b[2]), max_L
+
+
+def test_served_with_seeds_equals_seeded_searches(served):
+    """With 32 k-means start points: each served request equals the seeded
+    uneven-batch search and the seeded FAST search of its rows (same L, k),
+    and the seeds raise recall@10 over the single entry at the same L."""
+    import torch
+    import oracle as O
+    N, ix, reqs = served
+    seeds = N.pick_seeds(ix, 32)
+    srv = N.Server(ix, slots=2, max_batch_queries=256, fast=FAST, max_L=256,
+                   start_points=seeds)
+    for rows, L, k in reqs[:12]:
+        t = srv.submit(rows, L, k)
+        ids, d, _ = srv.wait(t)
+        qpad = np.zeros((len(rows), ix.row_stride), np.uint8)
+        qpad[:, :rows.shape[1]] = rows
+        qd = N.to_device(qpad)
+        Ls = torch.full((len(rows),), L, dtype=torch.int32, device="cuda")
+        ks = torch.full((len(rows),), k, dtype=torch.int32, device="cuda")
+        ui, ud, _ = N.search_uneven(ix, qd, Ls, ks, L, k, fast=FAST, start_points=seeds)
+        fi, fd, _ = N.search(ix, qd, L, k, mode=N.SEARCH_FAST, fast=FAST, start_points=seeds)
+        assert (ids == ui.cpu().numpy().view(np.uint32)).all() and (d == ud.cpu().numpy()).all()
+        assert (ids == fi.cpu().numpy().view(np.uint32)).all() and (d == fd.cpu().numpy()).all()
+    srv.close()
+    # recall at a small beam, seeds against node 0
+    q = np.concatenate([r for r, _, _ in reqs])[:300]
+    qpad = np.zeros((len(q), ix.row_stride), np.uint8)
+    qpad[:, :q.shape[1]] = q
+    qd = N.to_device(qpad)
+    gt, _ = O.compgt(ix.host_rows, q, 10, O.U8, tie_mode=1)
+    Ls = torch.full((len(q),), 12, dtype=torch.int32, device="cuda")
+    ks = torch.full((len(q),), 10, dtype=torch.int32, device="cuda")
+    a, _, _ = N.search_uneven(ix, qd, Ls, ks, 12, 10, fast=FAST, start_points=seeds)
+    b, _, _ = N.search_uneven(ix, qd, Ls, ks, 12, 10, fast=FAST)
+    ra = O.recall(a.cpu().numpy().view(np.uint32), gt, ix.host_rows, q, O.U8, 10)
+    rb = O.recall(b.cpu().numpy().view(np.uint32), gt, ix.host_rows, q, O.U8, 10)
+    assert ra >= rb - 0.01, (ra, rb)
+    with pytest.raises(N.NbpgError):
+        N.Server(ix, fast=FAST, max_L=300, start_points=seeds)
