@@ -5,18 +5,21 @@
 // per chunk of queries, all on one stream with no host synchronisation and
 // no allocation once the per-device workspace has grown to the call's size:
 //   1. threshold: v_mfma_i32_16x16x64_i8 distance tiles over S base rows
-//      sampled at a fixed stride.  Each lane keeps, per query, the two
+//      sampled at a fixed stride.  Each lane keeps, per query, the MF_TV (2)
 //      smallest distances of the rows it sees (a quarter of each 16-row
 //      block); a small kernel takes the J-th smallest of all those values as
 //      the query's bound U.  They are distances of distinct rows, so at
-//      least J rows lie within U, and about J * stride rows of the base.
+//      least J rows lie within U, and about J * stride rows of the base
+//      (rarely more: a lane would have to see more than two of the query's
+//      J nearest sample rows).
 //   2. scan: the same tiles over every base row; a row is kept when its
 //      distance is within U.  The row term enters the matrix core as the
 //      accumulator's start value, so the common case per 16 x 16 tile is
 //      one compare on the largest of a lane's four values and one
 //      wave-wide branch.  Kept rows go to a segment of their own per
 //      (query, base slice, lane), counted in a register: no atomics and no
-//      memory round trip in the scan.
+//      memory round trip in the scan.  A kept row is one 64-bit store of
+//      (partial distance as an order-preserving unsigned, row id).
 //   3. select: per query, its segments gathered and bitonic-sorted.  They
 //      hold every row within U, so when they hold at least k rows (and no
 //      segment overflowed) the first k are the exact top-k, ties at the
@@ -47,7 +50,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 #ifndef MF_BLK
 #define MF_BLK 128 // base rows per block staged through LDS
 #endif
-#define MF_J 16 // bound = the J-th smallest of the per-lane top-2 sample values
+#define MF_J 16 // bound = the J-th smallest of the per-lane top-MF_TV sample values
+#define MF_TV 2 // smallest sample distances each lane keeps per query
 #define MF_QCHUNK 16384
 #define MF_MAX_TSLICES 64
 
@@ -104,7 +108,7 @@ struct MfArgs {
     uint64_t *cand;      // scan: segments of seg_cap keys per (query, slice, lane group)
     uint32_t *segcnt;    // scan: rows kept per segment (beyond seg_cap: overflowed)
     uint32_t seg_cap;
-    int32_t *tv;         // threshold pass: per query and slice, 4 lanes x 2 values
+    int32_t *tv;         // threshold pass: per query and slice, 4 lanes x MF_TV values
 };
 
 // Distances are carried as the partial distance e = d - |q'|^2 (L2:
@@ -129,8 +133,9 @@ void mf_scan_kernel(MfArgs a)
     // 16 bytes at k-chunk kq), the B operand 16 queries (lane: query `col`,
     // the same chunk), and lane l holds D[4 kq + r][col], r = 0..3
     v4i qf[MF_QW][KS];
-    int32_t V[MF_QW], m1[MF_QW], m2[MF_QW];
+    int32_t V[MF_QW], mt[MF_QW][MF_TV]; // threshold: ascending
     uint32_t kept[MF_QW]; // scan: rows this lane kept per query tile
+    uint32_t segb[MF_QW]; // scan: first key of this lane's segment per query tile
 #pragma unroll
     for (int t = 0; t < MF_QW; ++t) {
         const uint64_t q = q0 + 16 * t + col;
@@ -149,9 +154,12 @@ void mf_scan_kernel(MfArgs a)
         // -U (inner product); never true for a missing query
         const int32_t u = q < a.nq ? a.U[q] : 0;
         V[t] = THRESH || q >= a.nq ? INT_MAX : (MIPS ? -u : -(u >> 1));
-        m1[t] = INT_MAX;
-        m2[t] = INT_MAX;
+#pragma unroll
+        for (int i = 0; i < MF_TV; ++i)
+            mt[t][i] = INT_MAX;
         kept[t] = 0;
+        // the host keeps every segment key index below 2^32
+        segb[t] = (uint32_t)(((q * gridDim.y + blockIdx.y) * 4 + kq) * a.seg_cap);
     }
     const uint64_t r0 = (uint64_t)blockIdx.y * a.slice_rows;
     const uint64_t r1 = min(a.rows, r0 + a.slice_rows);
@@ -242,9 +250,14 @@ void mf_scan_kernel(MfArgs a)
                 for (int t = 0; t < MF_QW; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int32_t e = MIPS ? w4[r] - c[t][r] : w4[r] - 2 * c[t][r];
-                        m2[t] = min(m2[t], max(m1[t], e));
-                        m1[t] = min(m1[t], e);
+                        // insert e into the ascending MF_TV smallest
+                        int32_t e = MIPS ? w4[r] - c[t][r] : w4[r] - 2 * c[t][r];
+#pragma unroll
+                        for (int i = 0; i < MF_TV; ++i) {
+                            const int32_t lo = min(mt[t][i], e);
+                            e = max(mt[t][i], e);
+                            mt[t][i] = lo;
+                        }
                     }
             } else {
                 bool hit = false;
@@ -256,19 +269,19 @@ void mf_scan_kernel(MfArgs a)
                     for (int t = 0; t < MF_QW; ++t) {
                         // only the lanes of tiles with a kept row
                         if (max(max(c[t][0], c[t][1]), max(c[t][2], c[t][3])) >= V[t]) {
-                            const uint64_t seg =
-                                ((q0 + 16 * t + col) * gridDim.y + blockIdx.y) * 4 + kq;
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                                 if (c[t][r] >= V[t]) {
-                                    // the exact partial distance from c' and w:
-                                    // e = w - 2c = (w & 1) - 2c' (L2), -c' (IP)
-                                    const int32_t w = w_lds[buf][16 * blk + 4 * kq + r];
-                                    const int32_t e = MIPS ? -c[t][r] : (w & 1) - 2 * c[t][r];
-                                    const uint64_t row = b0 + 16 * blk + 4 * kq + r;
+                                    // the partial distance from c': e = -c' (IP);
+                                    // e = w - 2c = (w & 1) - 2c' (L2), kept here
+                                    // as -2c' and completed by the select
+                                    // with the row's w & 1.  Key: the value as
+                                    // an order-preserving unsigned, the row id
+                                    const int32_t x = MIPS ? -c[t][r] : -2 * c[t][r];
+                                    const uint32_t row = (uint32_t)b0 + 16 * blk + 4 * kq + r;
                                     if (kept[t] < a.seg_cap)
-                                        a.cand[seg * a.seg_cap + kept[t]] =
-                                            make_key((float)e, (uint32_t)row);
+                                        a.cand[segb[t] + kept[t]] =
+                                            ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) | row;
                                     ++kept[t];
                                 }
                         }
@@ -315,9 +328,10 @@ void mf_scan_kernel(MfArgs a)
         for (int t = 0; t < MF_QW; ++t) {
             const uint64_t q = q0 + 16 * t + col;
             if (q < a.nq) {
-                int32_t *o = a.tv + ((q * slices + blockIdx.y) * 4 + kq) * 2;
-                o[0] = m1[t];
-                o[1] = m2[t];
+                int32_t *o = a.tv + ((q * slices + blockIdx.y) * 4 + kq) * MF_TV;
+#pragma unroll
+                for (int i = 0; i < MF_TV; ++i)
+                    o[i] = mt[t][i];
             }
         }
     }
@@ -336,7 +350,7 @@ __global__ __launch_bounds__(256) void mf_bound_kernel(const int32_t *__restrict
         *redo_cnt = 0;
     if (q >= nq)
         return;
-    constexpr int PER = 8 * MF_MAX_TSLICES / WAVE;
+    constexpr int PER = 4 * MF_TV * MF_MAX_TSLICES / WAVE;
     int32_t v[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
@@ -380,11 +394,14 @@ __global__ __launch_bounds__(256) void mf_bound_kernel(const int32_t *__restrict
 // per query: gather its nseg segments, sort the kept rows by (partial
 // distance, id), write the first k with their distances.  A query with an
 // overflowed segment, fewer than k kept rows or more than the workgroup
-// sorts goes on the redo list.
+// sorts goes on the redo list.  L2 keys arrive without the parity bit of
+// their partial distance (w & 1, w = |x'|^2 from nx), added back here
+// before the sort.
 template <bool MIPS>
 __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restrict__ cand,
                                                        const uint32_t *__restrict__ segcnt,
                                                        uint32_t nseg, uint32_t seg_cap,
+                                                       const uint32_t *__restrict__ nx,
                                                        const uint32_t *__restrict__ nqv,
                                                        uint32_t k, uint32_t *__restrict__ ids,
                                                        float *__restrict__ dists,
@@ -447,10 +464,17 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
                 for (uint32_t u = 0; u < 4; ++u)
                     if (e0 + u < c[j])
                         v[u] = cand[(q * nseg + i) * seg_cap + e0 + u];
+                uint32_t par[4] = {0, 0, 0, 0};
+                if (!MIPS)
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        if (e0 + u < c[j])
+                            par[u] = nx[(uint32_t)v[u]] & 1u;
+                // (x ^ 2^31) + 1 = (x + 1) ^ 2^31: |e| < 2^24, no carry out
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
                     if (e0 + u < c[j])
-                        s[start + e0 + u] = v[u];
+                        s[start + e0 + u] = v[u] + ((uint64_t)par[u] << 32);
             }
         start += c[j];
     }
@@ -479,7 +503,7 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
     for (uint32_t i = tid; i < k; i += 256) {
         ids[q * k + i] = key_id(s[i]);
         if (dists) {
-            const int32_t e = (int32_t)key_dist(s[i]);
+            const int32_t e = (int32_t)((uint32_t)(s[i] >> 32) ^ 0x80000000u);
             dists[q * k + i] = (float)(MIPS ? e - qn : e + qn);
         }
     }
@@ -616,6 +640,9 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     const uint64_t seg_keys = std::max<uint64_t>(chunk * gc.slices * 4 * gc.seg_cap,
                                                  last * gl.slices * 4 * gl.seg_cap);
     const uint64_t segs = std::max<uint64_t>(chunk * gc.slices, last * gl.slices) * 4;
+    if (seg_keys >= (1ull << 32))
+        return set_error(NBPG_EUNSUP, "MFMA ground truth: k = %u needs %llu candidate slots "
+                         "per query chunk (at most 2^32)", k, (unsigned long long)seg_keys);
     // workspace: nx | qn | U | redo count + list | tv | segment counts |
     // segments | the redo's partial buffer (for a full and for the last chunk)
     size_t off[9];
@@ -624,7 +651,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     off[2] = off[1] + align256(chunk * 4);
     off[3] = off[2] + align256(chunk * 4);
     off[4] = off[3] + align256((chunk + 1) * 4);
-    off[5] = off[4] + align256(chunk * 8 * MF_MAX_TSLICES * 4);
+    off[5] = off[4] + align256(chunk * 4 * MF_TV * MF_MAX_TSLICES * 4);
     off[6] = off[5] + align256(segs * 4);
     off[7] = off[6] + align256(seg_keys * 8);
     off[8] = off[7] + align256(std::max(compgt_plain_partial_bytes(chunk, n, k, true),
@@ -671,7 +698,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         a.slice_rows = mf_slice_rows(S, g.tiles, mf_per_cu(thr_fn), 1, MF_MAX_TSLICES);
         const uint32_t tslices = (uint32_t)((S + a.slice_rows - 1) / a.slice_rows);
         NBPG_TRY(mf_launch(thr_fn, dim3((uint32_t)g.tiles, tslices), s, a));
-        const uint32_t nv = 8 * tslices;
+        const uint32_t nv = 4 * MF_TV * tslices;
         hipLaunchKernelGGL(mf_bound_kernel, dim3((uint32_t)((m + 3) / 4)), dim3(256), 0, s, tv, nv,
                            std::min<uint32_t>(MF_J, nv), m, U, redo_cnt);
         // 2. scan
@@ -682,12 +709,12 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         // 3. select
         if (im)
             hipLaunchKernelGGL(mf_select_kernel<true>, dim3((uint32_t)m), dim3(256), 0, s, cand,
-                               segcnt, 4 * g.slices, g.seg_cap, nqv, k, ids, dists, redo_idx,
-                               redo_cnt);
+                               segcnt, 4 * g.slices, g.seg_cap, nx, nqv, k, ids, dists,
+                               redo_idx, redo_cnt);
         else
             hipLaunchKernelGGL(mf_select_kernel<false>, dim3((uint32_t)m), dim3(256), 0, s, cand,
-                               segcnt, 4 * g.slices, g.seg_cap, nqv, k, ids, dists, redo_idx,
-                               redo_cnt);
+                               segcnt, 4 * g.slices, g.seg_cap, nx, nqv, k, ids, dists,
+                               redo_idx, redo_cnt);
         if (hipGetLastError() != hipSuccess)
             return set_error(NBPG_EHIP, "MFMA ground-truth launch");
         // 4. the listed queries by the plain scan
