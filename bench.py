@@ -63,12 +63,13 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device (HBM counter calibration)")
     ap.add_argument("--workload", default="search",
-                    choices=["search", "c1", "pca", "uneven", "serve", "refgraph"],
+                    choices=["search", "c1", "pca", "uneven", "serve", "refgraph", "traffic"],
                     help="search: the headline 1M x 128 u8 batched search (default); c1: 100k x "
                          "128 u8 reference-exact build + search; pca: 1M x 960 f32 PCACompress + "
                          "search on codes; uneven: uneven batches + batched RobustPrune; serve: "
                          "10M x 96 i8 continuous-batching server under Poisson load; refgraph: "
-                         "the headline search on the graph the reference's own Vamana builds")
+                         "the headline search on the graph the reference's own Vamana builds; traffic: "
+                         "every FAST launch shape the autotuner can pick, for HBM counter runs")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

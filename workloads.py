@@ -1,4 +1,4 @@
-"""The non-headline workloads of bench.py (--workload c1 | pca | uneven | serve).
+"""The non-headline workloads of bench.py (--workload c1 | pca | uneven | serve | refgraph | traffic).
 
 Each run_* function sets up its workload on the device, measures it, checks
 it against the reference or the oracle where that is cheap, and returns one
@@ -345,6 +345,65 @@ def run_pca(args, log):
             _, yp = N.pca_project(x[:m], W, b, code=N.CODE_F32, mode=N.PCA_PLAIN)
             res["plain_fp32_equals_reference_y"] = bool((yp.cpu().numpy() == y).all())
     return res
+
+
+# ---------------------------------------------------------------------- traffic
+def run_traffic(args, log):
+    """HBM traffic per launch of every FAST launch shape the headline's
+    autotuner can pick, for a rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE run:
+    the headline's data, device-built graph, 64 k-means start points and 12
+    query batches; per shape (waves per group, expansion width, filter size,
+    row split; groups = as many as the device holds) the smallest beam width
+    with recall@10 >= 0.95 on batch 0, then, after every shape's beam width
+    is chosen, one eager search of each batch per shape in the order listed
+    in "launches" -- the last len(launches) x batches search dispatches of
+    the run, one per (shape, batch)."""
+    torch = _torch()
+    import itertools
+    import nbpg_amd as N
+    from nbpg_amd import autotune as AT
+    from nbpg_amd import datagen as D
+    N.init(0)
+    n, nq = args.n or 1_000_000, args.nq or 10_000
+    base, q = D.workload_1m(n, nq)
+    ix = N.Index(N.U8, n, base.shape[1], 64)
+    ix.set_points(base)
+    N.build(ix, 64, 70, VL=140, alpha=1.1, min_alpha=1.05, passes=2, num_split=4,
+            dynamic_prune=True, sp_seed=1)
+    base_d = N.to_device(base)
+    B = max(1, args.batches)
+    q_ds = [N.to_device(pad_rows(q, ix.row_stride))]
+    for b in range(1, B):
+        q_ds.append(N.to_device(pad_rows(D.workload_1m_queries(nq, b, q.shape[1]),
+                                         ix.row_stride)))
+    gt_ids, gt_d = N.compgt(base_d, torch.from_numpy(q).cuda(), 100, N.U8, N.GT_MFMA)
+    gt, gtd = gt_ids.cpu().numpy().view(np.uint32), gt_d.cpu().numpy()
+    seeds = N.pick_seeds(ix, 64)
+    sp = AT.DEFAULT_SPACE
+    shapes = []
+    for wpg, w, fl, rs in itertools.product(sp["waves_per_group"], sp["expand_width"],
+                                            sp["filter_log2"], sp["row_split"]):
+        cfg = (wpg, w, fl, 0, rs)
+        for L in AT.DEFAULT_L:
+            if L < 10:
+                continue
+            ids = N.search(ix, q_ds[0], L, 10, mode=N.SEARCH_FAST, fast=cfg, stats=False,
+                           start_points=seeds)[0]
+            r = AT.recall_with_ties(ids.cpu().numpy().view(np.uint32), gt, gtd, 10)
+            if r >= args.target:
+                shapes.append((cfg, L))
+                break
+    torch.cuda.synchronize()
+    log("traffic: %d shapes, %d launches each" % (len(shapes), B))
+    launches = []
+    for cfg, L in shapes:
+        for b in range(B):
+            N.search(ix, q_ds[b], L, 10, mode=N.SEARCH_FAST, fast=cfg, start_points=seeds)
+        torch.cuda.synchronize()
+        launches.append({"fast_config": list(cfg), "L": L})
+    ix.close()
+    return {"workload": "traffic", "n": n, "nq": nq, "dims": int(base.shape[1]),
+            "query_batches": B, "launches": launches}
 
 
 # ---------------------------------------------------------------------- refgraph
