@@ -60,6 +60,10 @@ def main():
                     help="FAST-mode start nodes from k-means over the base (1 = node 0 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served leg (the continuous-batching server on the same index)")
+    ap.add_argument("--serve-seconds", type=float, default=1.5,
+                    help="seconds of request traffic per offered load in the served leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device (HBM counter calibration)")
     ap.add_argument("--workload", default="search",
@@ -356,6 +360,27 @@ def main():
     plain_qps = nq * args.plain_steps / pwall
     log("plain L=%d, %d streams: %.0f QPS" % (L_plain, S, plain_qps))
 
+    # served: the continuous-batching server (nbpg_server_*) on the same
+    # index, beam width, launch shape and start points, with requests of
+    # 1..64 queries (k = 10) arriving as Poisson traffic at two offered loads
+    # and in a closed loop; latency is submit to completion per request,
+    # host staging and result copies included
+    served = None
+    if not args.no_serve:
+        import workloads as WL  # noqa: E402  (bench-only module beside this file)
+        srv = N.Server(ix, slots=3, max_batch_queries=4096, mem_budget=256 << 20,
+                       max_L=max(64, L_fast), max_k=10, fast=tuple(fast[:3]) + (0,) + tuple(fast[4:]),
+                       start_points=seeds)
+        loads = WL.serve_loads(srv, q, gt, gtd, L_fast, [0.0025 * qps, 0.01 * qps, None],
+                               args.serve_seconds, log)
+        srv.close()
+        served = {"L": L_fast, "k": 10, "slots": 3, "request_queries": "1..64",
+                  "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
+                  else "node 0", "loads": loads,
+                  "what": "the server on this index: Poisson arrivals at 0.25% and 1% of the "
+                          "batched value, then a closed loop (the Python client's submit rate "
+                          "bounds the offered load); latency = submit to completion"}
+
     # roofline of the fast kernel: algorithmic bytes of one launch from each
     # query's own counters -- expansions, distance computations and the
     # adjacency entries its expanded nodes hold (NBPG_SEARCH_EDGES):
@@ -449,6 +474,7 @@ def main():
                        "p99_ms": round(float(np.percentile(e2e, 99)), 4),
                        "what": "query rows from pinned host memory, search, ids and distances "
                                "back to pinned host memory, synchronised per batch (host clock)"},
+        "served": served,
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
                   "streams": S, "speedup_fast_over_plain": round(qps / plain_qps, 2)},
         "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)}

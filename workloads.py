@@ -694,21 +694,34 @@ def run_serve(args, log):
                                    for t, v in picks.items()}}
     srv = N.Server(ix, slots=3, max_batch_queries=4096, mem_budget=256 << 20, max_L=256,
                    max_k=10, fast=fast, start_points=sp_use)
-    rng = np.random.default_rng(5)
+    loads = serve_loads(srv, q, gt, gtd, L_pick, [0.01 * full_qps, 0.04 * full_qps, None], 4.0,
+                        log)
+    srv.close()
+    res["loads"] = loads
+    ix.close()
+    return res
+
+
+def serve_loads(srv, q, gt, gtd, L, targets, duration, log, k=10, seed=5):
+    """Drive a running nbpg server with requests of 1..64 consecutive rows
+    of q (k results each, beam width L) for `duration` seconds per target:
+    Poisson arrivals at each offered QPS in `targets` (queries per second),
+    or, for a target of None, a closed loop with 256 requests outstanding.
+    One row per target: offered / submitted / served QPS, the server's
+    p50 / p99 / max request latency (submit to completion), mean batch size
+    and recall@k of the served results against (gt, gtd)."""
+    rng = np.random.default_rng(seed)
+    nq = q.shape[0]
     sizes = rng.integers(1, 65, 400_000)
     starts = rng.integers(0, nq - 64, 400_000)
     loads = []
-    duration = 4.0
-    for offered in (0.01, 0.04, None):
-        # open loop: Poisson arrivals at a fraction of the whole-batch QPS;
-        # None: closed loop with 256 requests outstanding (saturation)
+    for target in targets:
         srv.stats(reset=True)
-        qps_target = (offered or 1.0) * full_qps
-        gaps = rng.exponential(32.5 / qps_target, 400_000)
+        gaps = rng.exponential(32.5 / (target or 1.0), 400_000)
         tickets = []
         done = threading.Event()
         results = {}
-        window = threading.Semaphore(256) if offered is None else None
+        window = threading.Semaphore(256) if target is None else None
 
         def consumer():
             i = 0
@@ -745,7 +758,7 @@ def run_serve(args, log):
                         while time.perf_counter() < nxt:
                             pass
                 s0, m = int(starts[i]), int(sizes[i])
-                tickets.append((srv.submit(q[s0:s0 + m], L_pick, 10), s0, m))
+                tickets.append((srv.submit(q[s0:s0 + m], L, k), s0, m))
                 i += 1
         finally:
             submit_end = time.perf_counter()
@@ -756,17 +769,14 @@ def run_serve(args, log):
         got = [j for j in range(len(tickets)) if j in results]
         ids = np.concatenate([results[j][2] for j in got])
         qi = np.concatenate([np.arange(results[j][0], results[j][0] + results[j][1]) for j in got])
-        rec = recall(ids, gt[qi], gtd[qi], 10)
-        row = {"offered_qps": round(qps_target, 1) if offered else "closed loop, 256 requests "
+        rec = recall(ids, gt[qi], gtd[qi], k)
+        row = {"offered_qps": round(target, 1) if target else "closed loop, 256 requests "
                "outstanding", "submitted_qps": round(st["queries"] / (submit_end - t0), 1),
                "served_qps": round(st["queries"] / wall, 1), "requests": st["requests"],
                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                "max_ms": round(st["max_ms"], 3),
                "mean_batch_queries": round(st["mean_batch_queries"], 1),
-               "recall@10": round(rec, 4)}
+               "recall@%d" % k: round(rec, 4)}
         loads.append(row)
         log("serve", row)
-    srv.close()
-    res["loads"] = loads
-    ix.close()
-    return res
+    return loads
