@@ -30,7 +30,9 @@
 #include <random>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "prune_core.h"
 #include "ref_search.h"
@@ -363,8 +365,8 @@ struct BuildCtx {
     uint32_t *tgtA = nullptr, *tgtB = nullptr, *uniq = nullptr, *counts = nullptr,
              *offsets = nullptr, *num_runs = nullptr;
     uint64_t *keyA = nullptr, *keyB = nullptr;
-    void *cub_tmp = nullptr;
-    size_t cub_bytes = 0;
+    void *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
     float *popular = nullptr, *alphas = nullptr, *coss = nullptr;
     uint8_t *gscratch = nullptr;
     uint64_t wave_bytes = 0;
@@ -388,7 +390,7 @@ static void build_free(BuildCtx &c)
     hipFree(c.num_runs);
     hipFree(c.keyA);
     hipFree(c.keyB);
-    hipFree(c.cub_tmp);
+    hipFree(c.sort_tmp);
     hipFree(c.popular);
     hipFree(c.alphas);
     hipFree(c.coss);
@@ -538,19 +540,20 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
         hipLaunchKernelGGL(edges_kernel, dim3((uint32_t)((ne + 255) / 256)), dim3(256), 0, c.s,
                            c.d_inserts, fl, bs, c.p_ids, c.p_d, c.p_cnt, c.R, c.tgtA, c.keyA);
         // (dist, source) then stable by target
-        size_t tb = c.cub_bytes;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c.cub_tmp, tb, c.keyA, c.keyB, c.tgtA, c.tgtB,
-                                                   (int)ne, 0, 64, c.s));
-        tb = c.cub_bytes;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c.cub_tmp, tb, c.tgtB, c.tgtA, c.keyB, c.keyA,
-                                                   (int)ne, 0, 32, c.s));
+        size_t tb = c.sort_bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(c.sort_tmp, tb, c.keyA, c.keyB, c.tgtA, c.tgtB, ne, 0u,
+                                          64u, c.s));
+        tb = c.sort_bytes;
+        HIP_TRY(rocprim::radix_sort_pairs(c.sort_tmp, tb, c.tgtB, c.tgtA, c.keyB, c.keyA, ne, 0u,
+                                          32u, c.s));
         hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((ne + 255) / 256)), dim3(256), 0, c.s,
                            c.counts, ne, 0u);
-        tb = c.cub_bytes;
-        HIP_TRY(hipcub::DeviceRunLengthEncode::Encode(c.cub_tmp, tb, c.tgtA, c.uniq, c.counts,
-                                                      c.num_runs, (int)ne, c.s));
-        tb = c.cub_bytes;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c.cub_tmp, tb, c.counts, c.offsets, (int)ne, c.s));
+        tb = c.sort_bytes;
+        HIP_TRY(rocprim::run_length_encode(c.sort_tmp, tb, c.tgtA, (unsigned int)ne, c.uniq,
+                                           c.counts, c.num_runs, c.s));
+        tb = c.sort_bytes;
+        HIP_TRY(rocprim::exclusive_scan(c.sort_tmp, tb, c.counts, c.offsets, 0u, (size_t)ne,
+                                        rocprim::plus<uint32_t>(), c.s));
         ra.max_groups = ne;
         rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
             hipLaunchKernelGGL((reverse_kernel<DT.value, NCH.value>),
@@ -643,15 +646,14 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
     }
     if (ok) {
         size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-        hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c.keyA, c.keyB, c.tgtA, c.tgtB, (int)ne, 0,
-                                           64, c.s);
-        hipcub::DeviceRadixSort::SortPairs(nullptr, b2, c.tgtB, c.tgtA, c.keyB, c.keyA, (int)ne, 0,
-                                           32, c.s);
-        hipcub::DeviceRunLengthEncode::Encode(nullptr, b3, c.tgtA, c.uniq, c.counts, c.num_runs,
-                                              (int)ne, c.s);
-        hipcub::DeviceScan::ExclusiveSum(nullptr, b4, c.counts, c.offsets, (int)ne, c.s);
-        c.cub_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
-        ok = hipMalloc(&c.cub_tmp, c.cub_bytes) == hipSuccess;
+        rocprim::radix_sort_pairs(nullptr, b1, c.keyA, c.keyB, c.tgtA, c.tgtB, ne, 0u, 64u, c.s);
+        rocprim::radix_sort_pairs(nullptr, b2, c.tgtB, c.tgtA, c.keyB, c.keyA, ne, 0u, 32u, c.s);
+        rocprim::run_length_encode(nullptr, b3, c.tgtA, (unsigned int)ne, c.uniq, c.counts,
+                                   c.num_runs, c.s);
+        rocprim::exclusive_scan(nullptr, b4, c.counts, c.offsets, 0u, (size_t)ne,
+                                rocprim::plus<uint32_t>(), c.s);
+        c.sort_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
+        ok = hipMalloc(&c.sort_tmp, c.sort_bytes) == hipSuccess;
     }
     if (!ok) {
         build_free(c);
