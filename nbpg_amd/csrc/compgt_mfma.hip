@@ -44,9 +44,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 #ifndef MF_QW
 #define MF_QW 4 // 16-query tiles per wavefront (B operand, registers)
 #endif
-#ifndef MF_WAVES
-#define MF_WAVES 8 // wavefronts per workgroup: 512 queries share each staged block
-#endif
+// wavefronts per workgroup: 16 (1024 queries share each staged block) for
+// rows of up to 128 bytes, whose scan fits 128 VGPRs (4 waves per SIMD, one
+// workgroup per CU); 8 for wider rows, whose scan holds 2 waves per SIMD
+__host__ __device__ constexpr uint32_t mf_waves(uint32_t KS) { return KS <= 2 ? 16u : 8u; }
 #ifndef MF_BLK
 #define MF_BLK 128 // base rows per block staged through LDS
 #endif
@@ -116,7 +117,7 @@ struct MfArgs {
 // per query the order of e is the order of d.  Each block row's w holds
 // |x'|^2 or -A(x); a missing row has w = 2^30, beyond every bound.
 template <int KS, bool MIPS, bool THRESH>
-__global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(KS <= 2 ? 4 : 2)))
+__global__ __launch_bounds__(64 * mf_waves(KS)) __attribute__((amdgpu_waves_per_eu(KS <= 2 ? 4 : 2)))
 void mf_scan_kernel(MfArgs a)
 {
     constexpr uint32_t PITCH = 64 * KS + 16; // LDS row pitch (16-byte pad against bank conflicts)
@@ -126,7 +127,8 @@ void mf_scan_kernel(MfArgs a)
     __shared__ __align__(16) int32_t i_lds[2][MF_BLK];
     const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     const uint32_t col = lane % 16, kq = lane / 16;
-    const uint64_t q0 = ((uint64_t)blockIdx.x * MF_WAVES + wv) * (16 * MF_QW);
+    constexpr uint32_t WAVES = mf_waves(KS);
+    const uint64_t q0 = ((uint64_t)blockIdx.x * WAVES + wv) * (16 * MF_QW);
     const bool active = q0 < a.nq; // inactive waves still stage blocks and synchronise
     const uint32_t xmask = a.is_u8 ? 0x80808080u : 0u;
     // D = rows x queries: the A operand is a 16-row block (lane: row `col`,
@@ -163,11 +165,11 @@ void mf_scan_kernel(MfArgs a)
     }
     const uint64_t r0 = (uint64_t)blockIdx.y * a.slice_rows;
     const uint64_t r1 = min(a.rows, r0 + a.slice_rows);
-    // this thread's pieces of a block: piece p = tid + 64 MF_WAVES j is row
+    // this thread's pieces of a block: piece p = tid + 64 WAVES j is row
     // p / (4 KS), 16-byte chunk p % (4 KS).  Two register sets: a block is
     // loaded two blocks ahead of its use, so its load has two blocks' worth
     // of scoring to arrive before it is written to LDS.
-    constexpr uint32_t PIECES = (MF_BLK * 4 * KS + 64 * MF_WAVES - 1) / (64 * MF_WAVES);
+    constexpr uint32_t PIECES = (MF_BLK * 4 * KS + 64 * WAVES - 1) / (64 * WAVES);
     v4i stage[2][PIECES];
     int32_t wstage[2] = {MISSING, MISSING};
     const uint64_t row_bytes = a.row_step * a.dims;
@@ -180,7 +182,7 @@ void mf_scan_kernel(MfArgs a)
         const uint64_t left = b0 < r1 ? r1 - b0 : 0; // rows of this block that exist
 #pragma unroll
         for (uint32_t j = 0; j < PIECES; ++j) {
-            const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
+            const uint32_t p = threadIdx.x + 64 * WAVES * j;
             const uint32_t prow = p / (4 * KS), off = 16 * (p % (4 * KS));
             v4i v = {0, 0, 0, 0};
             if (p < MF_BLK * 4 * KS && off < a.dims && prow < left) {
@@ -203,7 +205,7 @@ void mf_scan_kernel(MfArgs a)
     auto put = [&](int buf, auto R) {
 #pragma unroll
         for (uint32_t j = 0; j < PIECES; ++j) {
-            const uint32_t p = threadIdx.x + 64 * MF_WAVES * j;
+            const uint32_t p = threadIdx.x + 64 * WAVES * j;
             if (p < MF_BLK * 4 * KS)
                 *reinterpret_cast<v4i *>(&rows_lds[buf][(p / (4 * KS)) * PITCH + 16 * (p % (4 * KS))]) =
                     stage[R.value][j];
@@ -559,7 +561,7 @@ template <bool THRESH> static const void *mf_kernel(uint32_t KS, bool mips)
     }
 }
 
-static uint32_t mf_per_cu(const void *fn)
+static uint32_t mf_per_cu(const void *fn, uint32_t waves)
 {
     static std::mutex mu;
     static std::vector<std::pair<const void *, uint32_t>> cache;
@@ -568,7 +570,7 @@ static uint32_t mf_per_cu(const void *fn)
         if (e.first == fn)
             return e.second;
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64 * MF_WAVES, 0) != hipSuccess || b < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64 * waves, 0) != hipSuccess || b < 1)
         b = 1;
     cache.emplace_back(fn, (uint32_t)b);
     return (uint32_t)b;
@@ -586,10 +588,10 @@ static uint64_t mf_slice_rows(uint64_t rows, uint64_t tiles, uint32_t per_cu, ui
     return ((rows + slices - 1) / slices + MF_BLK - 1) / MF_BLK * MF_BLK;
 }
 
-static int mf_launch(const void *fn, dim3 grid, hipStream_t s, MfArgs &a)
+static int mf_launch(const void *fn, dim3 grid, uint32_t waves, hipStream_t s, MfArgs &a)
 {
     void *args[] = {&a};
-    if (hipLaunchKernel(fn, grid, dim3(64 * MF_WAVES), args, 0, s) != hipSuccess)
+    if (hipLaunchKernel(fn, grid, dim3(64 * waves), args, 0, s) != hipSuccess)
         return set_error(NBPG_EHIP, "MFMA ground-truth scan launch");
     return NBPG_OK;
 }
@@ -616,7 +618,8 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     const uint64_t nchunks = (nq + MF_QCHUNK - 1) / MF_QCHUNK;
     const uint64_t chunk = (nq + nchunks - 1) / nchunks;
     const uint64_t last = nq - (nq - 1) / chunk * chunk;
-    const uint64_t qpb = 16ull * MF_QW * MF_WAVES;
+    const uint32_t waves = mf_waves(KS);
+    const uint64_t qpb = 16ull * MF_QW * waves;
     const void *thr_fn = mf_kernel<true>(KS, im), *scan_fn = mf_kernel<false>(KS, im);
     // scan geometry of a chunk of m queries: slices of the base, and the
     // segment size for about 4k kept rows per query spread over 4 segments
@@ -628,7 +631,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
     auto geo = [&](uint64_t m) {
         Geo g;
         g.tiles = (m + qpb - 1) / qpb;
-        g.slice_rows = mf_slice_rows(n, g.tiles, mf_per_cu(scan_fn), 4, MF_SEL_SEGS / 4);
+        g.slice_rows = mf_slice_rows(n, g.tiles, mf_per_cu(scan_fn, waves), 4, MF_SEL_SEGS / 4);
         g.slices = (uint32_t)((n + g.slice_rows - 1) / g.slice_rows);
         const uint64_t want = std::max<uint64_t>(16, 3ull * k / g.slices + 16);
         g.seg_cap = 16;
@@ -695,9 +698,9 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         // 1. bound from the sample
         a.rows = S;
         a.row_step = step;
-        a.slice_rows = mf_slice_rows(S, g.tiles, mf_per_cu(thr_fn), 1, MF_MAX_TSLICES);
+        a.slice_rows = mf_slice_rows(S, g.tiles, mf_per_cu(thr_fn, waves), 1, MF_MAX_TSLICES);
         const uint32_t tslices = (uint32_t)((S + a.slice_rows - 1) / a.slice_rows);
-        NBPG_TRY(mf_launch(thr_fn, dim3((uint32_t)g.tiles, tslices), s, a));
+        NBPG_TRY(mf_launch(thr_fn, dim3((uint32_t)g.tiles, tslices), waves, s, a));
         const uint32_t nv = 4 * MF_TV * tslices;
         hipLaunchKernelGGL(mf_bound_kernel, dim3((uint32_t)((m + 3) / 4)), dim3(256), 0, s, tv, nv,
                            std::min<uint32_t>(MF_J, nv), m, U, redo_cnt);
@@ -705,7 +708,7 @@ int compgt_mfma(int dtype, const uint8_t *base, uint64_t n, const uint8_t *queri
         a.rows = n;
         a.row_step = 1;
         a.slice_rows = g.slice_rows;
-        NBPG_TRY(mf_launch(scan_fn, dim3((uint32_t)g.tiles, g.slices), s, a));
+        NBPG_TRY(mf_launch(scan_fn, dim3((uint32_t)g.tiles, g.slices), waves, s, a));
         // 3. select
         if (im)
             hipLaunchKernelGGL(mf_select_kernel<true>, dim3((uint32_t)m), dim3(256), 0, s, cand,
