@@ -274,6 +274,20 @@ int nbpg_search_uneven(const nbpg_index *idx, const void *d_queries, uint64_t nq
                        uint32_t max_L, uint32_t out_k, const uint32_t *h_start_points,
                        uint32_t sp_num, const nbpg_fast_config *fast, uint32_t *d_out_ids,
                        float *d_out_dists, uint32_t *d_out_stats, void *stream);
+/* nbpg_search_uneven captured as a HIP graph: the uneven batch of a
+ * qsearchAll-style driver (beamSearch.h:257-303) as one graph launch.  Same
+ * rules as nbpg_search_graph_*: create makes one eager call, then captures
+ * it; a launch reads whatever d_queries, d_L and d_k hold at that time (the
+ * per-query L and k may change between launches, capped by max_L and out_k
+ * fixed at creation) and writes the same outputs; the graph owns its work
+ * counter and holds no memset; launches of one graph must not overlap. */
+int nbpg_search_uneven_graph_create(nbpg_search_graph **out, const nbpg_index *idx,
+                                    const void *d_queries, uint64_t nq, uint64_t query_stride,
+                                    const uint32_t *d_L, const uint32_t *d_k, uint32_t max_L,
+                                    uint32_t out_k, const uint32_t *h_start_points,
+                                    uint32_t sp_num, const nbpg_fast_config *fast,
+                                    uint32_t *d_out_ids, float *d_out_dists,
+                                    uint32_t *d_out_stats);
 
 /* ---- RobustPrune: Prune::sort_candidates + Prune::robustPrune
  * (prune.h:49-87, 158-209) for a batch of nodes.  Node t's candidates are

@@ -96,6 +96,8 @@ _SIGS = {
                                        C.POINTER(QueryParams), I32, C.POINTER(FastConfig),
                                        VP, VP, VP]),
     "nbpg_search_graph_launch": (I32, [VP, VP]),
+    "nbpg_search_uneven_graph_create": (I32, [C.POINTER(VP), VP, VP, U64, U64, VP, VP, U32, U32,
+                                              VP, U32, C.POINTER(FastConfig), VP, VP, VP]),
     "nbpg_pick_seeds": (I32, [VP, U32, U32, U32, U64, VP, C.POINTER(U32)]),
     "nbpg_search_graph_destroy": (I32, [VP]),
     "nbpg_search": (I32, [VP, VP, U64, U64, VP, U32, C.POINTER(QueryParams), I32,
@@ -351,6 +353,29 @@ def search_rerank(index, q_dev, raw_base_dev, raw_q_dev, L, k, rerank_factor=100
                                 C.byref(fc) if fc is not None else None, _ptr(ids), _ptr(d),
                                 _ptr(st), _stream()), name)
     return ids, d, st
+
+
+class UnevenSearchGraph(SearchGraph):
+    """nbpg_search_uneven captured as a HIP graph (nbpg_search_uneven_graph_create):
+    replay() searches whatever q_dev, L_dev and k_dev hold into the same
+    outputs."""
+
+    def __init__(self, index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None,
+                 start_points=None):
+        torch = _torch()
+        nq = q_dev.shape[0]
+        qstride = q_dev.shape[1] * q_dev.element_size()
+        sp = np.array([entry] if start_points is None else start_points, dtype=np.uint32)
+        self.ids = torch.empty((nq, out_k), dtype=torch.int32, device="cuda")
+        self.dists = torch.empty((nq, out_k), dtype=torch.float32, device="cuda")
+        self.stats = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+        fc = FastConfig(*fast) if fast is not None else None
+        h = VP()
+        _check(lib().nbpg_search_uneven_graph_create(
+            C.byref(h), index.h, _ptr(q_dev), nq, qstride, _ptr(L_dev), _ptr(k_dev), max_L, out_k,
+            sp.ctypes.data, len(sp), C.byref(fc) if fc is not None else None, _ptr(self.ids),
+            _ptr(self.dists), _ptr(self.stats)), "nbpg_search_uneven_graph_create")
+        self.h, self.index, self.q_dev = h, index, q_dev
 
 
 def search_uneven(index, q_dev, L_dev, k_dev, max_L, out_k, entry=0, fast=None, out=None,

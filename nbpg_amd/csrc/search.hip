@@ -752,6 +752,48 @@ struct nbpg_search_graph {
     uint32_t *counter = nullptr; // the captured launch's own work counter
 };
 
+namespace nbpg {
+
+// captures launch(stream, counter) into a new graph with its own work
+// counter; launch is one search call that has already run eagerly
+template <typename F> static int capture_search(nbpg_search_graph **out, F &&launch)
+{
+    nbpg_search_graph *g = new nbpg_search_graph();
+    // the graph's own work counter, so replays never share one with other
+    // launches on the index
+    if (hipMalloc(&g->counter, 256) != hipSuccess || hipMemset(g->counter, 0, 256) != hipSuccess) {
+        nbpg_search_graph_destroy(g);
+        return set_error(NBPG_ENOMEM, "graph work counter");
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        nbpg_search_graph_destroy(g);
+        return set_error(NBPG_EHIP, "capture stream");
+    }
+    int rc = NBPG_OK;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = set_error(NBPG_EHIP, "graph capture start");
+    } else {
+        rc = launch(s, g->counter);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc == NBPG_OK && e != hipSuccess)
+            rc = set_error(NBPG_EHIP, "graph capture: %s", hipGetErrorString(e));
+        g->graph = graph;
+        if (rc == NBPG_OK && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = set_error(NBPG_EHIP, "graph instantiation");
+    }
+    hipStreamDestroy(s);
+    if (rc != NBPG_OK) {
+        nbpg_search_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return NBPG_OK;
+}
+
+} // namespace nbpg
+
 extern "C" {
 
 int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *ix, const void *d_queries,
@@ -771,39 +813,36 @@ int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *ix, cons
     NBPG_TRY(nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
                          d_out_ids, d_out_dists, d_out_stats, nullptr));
     HIP_TRY(hipDeviceSynchronize());
-    nbpg_search_graph *g = new nbpg_search_graph();
-    // the graph's own work counter, so replays never share one with other
-    // launches on the index
-    if (hipMalloc(&g->counter, 256) != hipSuccess || hipMemset(g->counter, 0, 256) != hipSuccess) {
-        nbpg_search_graph_destroy(g);
-        return set_error(NBPG_ENOMEM, "graph work counter");
-    }
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-        nbpg_search_graph_destroy(g);
-        return set_error(NBPG_EHIP, "capture stream");
-    }
-    int rc = NBPG_OK;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        rc = set_error(NBPG_EHIP, "graph capture start");
-    } else {
-        rc = search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
-                          d_out_ids, d_out_dists, d_out_stats, s, g->counter);
-        hipGraph_t graph = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &graph);
-        if (rc == NBPG_OK && e != hipSuccess)
-            rc = set_error(NBPG_EHIP, "graph capture: %s", hipGetErrorString(e));
-        g->graph = graph;
-        if (rc == NBPG_OK && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
-            rc = set_error(NBPG_EHIP, "graph instantiation");
-    }
-    hipStreamDestroy(s);
-    if (rc != NBPG_OK) {
-        nbpg_search_graph_destroy(g);
-        return rc;
-    }
-    *out = g;
-    return NBPG_OK;
+    return capture_search(out, [&](hipStream_t s, uint32_t *counter) {
+        return search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
+                            d_out_ids, d_out_dists, d_out_stats, s, counter);
+    });
+}
+
+int nbpg_search_uneven_graph_create(nbpg_search_graph **out, const nbpg_index *ix,
+                                    const void *d_queries, uint64_t nq, uint64_t query_stride,
+                                    const uint32_t *d_L, const uint32_t *d_k, uint32_t max_L,
+                                    uint32_t out_k, const uint32_t *h_start_points,
+                                    uint32_t sp_num, const nbpg_fast_config *fast,
+                                    uint32_t *d_out_ids, float *d_out_dists,
+                                    uint32_t *d_out_stats)
+{
+    if (!out)
+        return set_error(NBPG_EINVAL, "null output handle");
+    *out = nullptr;
+    NBPG_TRY(ensure_init());
+    if (nq == 0)
+        return set_error(NBPG_EINVAL, "empty batch");
+    // the eager call checks every argument the capture uses
+    NBPG_TRY(nbpg_search_uneven(ix, d_queries, nq, query_stride, d_L, d_k, max_L, out_k,
+                                h_start_points, sp_num, fast, d_out_ids, d_out_dists, d_out_stats,
+                                nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    return capture_search(out, [&](hipStream_t s, uint32_t *counter) {
+        return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points,
+                           sp_num, 0, 0, d_L, d_k, max_L, out_k, ix->n, fast, d_out_ids,
+                           d_out_dists, d_out_stats, counter, s);
+    });
 }
 
 int nbpg_search_graph_launch(nbpg_search_graph *g, void *stream)

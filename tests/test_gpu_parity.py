@@ -648,6 +648,43 @@ def test_search_graph_replay_equals_eager(gpu, small, mode, L):
     ix.close()
 
 
+def test_uneven_search_graph_replay_equals_eager(gpu, small):
+    """nbpg_search_uneven_graph_*: the captured uneven search replayed with new
+    query rows, beam widths and k written into the captured buffers between
+    replays (eager searches and synchronisations in between) equals the eager
+    nbpg_search_uneven of the same inputs every time."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    seeds = N.pick_seeds(ix, 16)
+    rng = np.random.default_rng(23)
+    qa = dev_rows(N, q[:100], ix.row_stride)
+    qb = dev_rows(N, q[100:200], ix.row_stride)
+
+    def lk(seed):
+        r = np.random.default_rng(seed)
+        L = np.exp(r.uniform(np.log(16), np.log(256), 100)).round().astype(np.uint32)
+        k = np.minimum(r.choice(np.array([1, 10, 100], np.uint32), 100), L)
+        return (torch.from_numpy(L.view(np.int32)).cuda(), torch.from_numpy(k.view(np.int32)).cuda())
+    qbuf, (Lbuf, kbuf) = qa.clone(), lk(1)
+    g = N.UnevenSearchGraph(ix, qbuf, Lbuf, kbuf, 256, 100, start_points=seeds)
+    for rep in range(4):
+        src = qa if rep % 2 == 0 else qb
+        Ls, ks = lk(int(rng.integers(1 << 30)))
+        qbuf.copy_(src)
+        Lbuf.copy_(Ls)
+        kbuf.copy_(ks)
+        got = [t.clone() for t in g.replay()]
+        torch.cuda.synchronize()
+        want = N.search_uneven(ix, src, Ls, ks, 256, 100, start_points=seeds)
+        torch.cuda.synchronize()
+        for a, b in zip(got, want):
+            assert torch.equal(a, b), rep
+    g.close()
+    ix.close()
+
+
 @pytest.mark.parametrize("kind", ["fast", "uneven"])
 def test_concurrent_searches_on_streams(gpu, small, kind):
     """Searches on one index issued round-robin on three streams, 70 of them

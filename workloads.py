@@ -39,18 +39,13 @@ def events_ms(fn, reps, warm=1):
     return a.elapsed_time(b) / reps
 
 
-def graphed(fn):
-    """fn captured once into a hipGraph; returns the replay callable."""
-    torch = _torch()
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        fn()
-    torch.cuda.current_stream().wait_stream(s)
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        fn()
-    return g.replay
+def replay_ms(graph, reps):
+    """Mean device time of a library search graph's replay (nbpg_search_graph_*
+    or the uneven form) over reps replays; the graph is closed afterwards."""
+    try:
+        return events_ms(graph.replay, reps)
+    finally:
+        graph.close()
 
 
 def pad_rows(a, stride):
@@ -166,8 +161,8 @@ def run_c1(args, log):
         pi = pi.cpu().numpy().view(np.uint32)
         plain_ms = events_ms(lambda: N.search(ix, q_d, L, 10, mode=N.SEARCH_REF), 5)
         out = N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=(2, 1, 11, 0, 0))
-        fast_ms = events_ms(graphed(lambda: N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST,
-                                                      fast=(2, 1, 11, 0, 0), out=out)), 20)
+        fast_ms = replay_ms(N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST,
+                                          fast=(2, 1, 11, 0, 0)), 20)
         fi = out[0].cpu().numpy().view(np.uint32)
         row = {"L": L, "plain_qps": round(nq / plain_ms * 1e3, 1),
                "fast_qps": round(nq / fast_ms * 1e3, 1),
@@ -294,8 +289,8 @@ def run_pca(args, log):
         rows = []
         for L in (16, 32, 64, 128):
             out = N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=(2, 1, 11, 0, 0))
-            ms = events_ms(graphed(lambda: N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST,
-                                                     fast=(2, 1, 11, 0, 0), out=out)), 10)
+            ms = replay_ms(N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST,
+                                         fast=(2, 1, 11, 0, 0)), 10)
             ids = out[0].cpu().numpy().view(np.uint32)
             rows.append({"L": L, "qps": round(nq / ms * 1e3, 1),
                          "recall@10_code_space": round(recall(ids, cgt, cgd, 10), 4),
@@ -525,16 +520,14 @@ def run_uneven(args, log):
     res = {"workload": "uneven", "n": n, "nq": nq, "start_points": "%d k-means seeds" % len(seeds)}
 
     def batch(idx, fast, sp=seeds):
+        """the batch's search captured by the library as one HIP graph
+        (nbpg_search_uneven_graph_create): (replay, graph)"""
         qd = N.to_device(qpad[idx])
         Ld = torch.from_numpy(Lq[idx]).cuda()
         kd = torch.from_numpy(kq[idx]).cuda()
         Lmax, kmax = int(Lq[idx].max()), int(kq[idx].max())
-        holder = {}
-
-        def run():
-            holder["out"] = N.search_uneven(ix, qd, Ld, kd, Lmax, kmax, fast=fast,
-                                            start_points=sp)
-        return run, holder
+        g = N.UnevenSearchGraph(ix, qd, Ld, kd, Lmax, kmax, fast=fast, start_points=sp)
+        return g.replay, g
 
     # autotune the launch shape on the 4096-query batch
     idx4k = rng.choice(nq, 4096, replace=False)
@@ -544,8 +537,9 @@ def run_uneven(args, log):
             for fl in (10, 11, 12):
                 fast = (wpg, W, fl, 0, 0)
                 try:
-                    run, _ = batch(idx4k, fast)
-                    ms = events_ms(graphed(run), 5)
+                    run, g = batch(idx4k, fast)
+                    ms = events_ms(run, 5)
+                    g.close()
                 except N.NbpgError:
                     continue
                 tune.append({"config": list(fast), "ms": round(ms, 4)})
@@ -558,11 +552,10 @@ def run_uneven(args, log):
     for B, sp in ((1, seeds), (16, seeds), (256, seeds), (1024, seeds), (4096, seeds),
                   (4096, None)):
         idx = rng.choice(nq, B, replace=False)
-        run, holder = batch(idx, fast, sp)
-        ms = events_ms(graphed(run), 20 if B <= 256 else 10)
-        run()
-        torch.cuda.synchronize()
-        ids = holder["out"][0].cpu().numpy().view(np.uint32)
+        run, g = batch(idx, fast, sp)
+        ms = events_ms(run, 20 if B <= 256 else 10)
+        ids = run()[0].cpu().numpy().view(np.uint32)
+        g.close()
         rec = []
         for j, qi in enumerate(idx):
             k = int(kq[qi])
@@ -675,8 +668,8 @@ def run_serve(args, log):
             r = recall(out[0].cpu().numpy().view(np.uint32), gt, gtd, 10)
             log("serve %s L=%d recall %.4f" % (tag, L, r))
             if r >= 0.95:
-                ms = events_ms(graphed(lambda: N.search(ix, q_d, L, 10, mode=N.SEARCH_FAST,
-                                                         fast=fast, out=out, start_points=sp)), 10)
+                ms = replay_ms(N.SearchGraph(ix, q_d, L, 10, mode=N.SEARCH_FAST, fast=fast,
+                                             start_points=sp), 10)
                 picks[tag] = (L, r, nq / ms * 1e3)
                 break
     if not picks:
