@@ -371,15 +371,17 @@ def main():
         srv = N.Server(ix, slots=3, max_batch_queries=4096, mem_budget=256 << 20,
                        max_L=max(64, L_fast), max_k=10, fast=tuple(fast[:3]) + (0,) + tuple(fast[4:]),
                        start_points=seeds)
-        loads = WL.serve_loads(srv, q, gt, gtd, L_fast, [0.0025 * qps, 0.01 * qps, None],
+        loads = WL.serve_loads(srv, q, gt, gtd, L_fast,
+                               [0.0025 * qps, 0.01 * qps, None, ("burst", 64)],
                                args.serve_seconds, log)
         srv.close()
         served = {"L": L_fast, "k": 10, "slots": 3, "request_queries": "1..64",
                   "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
                   else "node 0", "loads": loads,
                   "what": "the server on this index: Poisson arrivals at 0.25% and 1% of the "
-                          "batched value, then a closed loop (the Python client's submit rate "
-                          "bounds the offered load); latency = submit to completion"}
+                          "batched value, a closed loop of single requests (bounded by the "
+                          "Python client's submit rate), and a closed loop submitting bursts "
+                          "of 64 requests per call; latency = submit to completion"}
 
     # roofline of the fast kernel: algorithmic bytes of one launch from each
     # query's own counters -- expansions, distance computations and the

@@ -383,6 +383,20 @@ int nbpg_server_submit(nbpg_server *srv, const void *host_queries, uint32_t nq,
  * waiting on returns NBPG_EINVAL. */
 int nbpg_server_wait(nbpg_server *srv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
                      double *latency_ms);
+/* a burst of n requests queued in one call (a front end that receives
+ * several at once): request i is req_nq[i] consecutive rows of host_queries,
+ * after request i-1's, all with the same L and k; tickets[i] as
+ * nbpg_server_submit gives.  All n are checked before any is queued. */
+int nbpg_server_submit_many(nbpg_server *srv, const void *host_queries, uint32_t n,
+                            const uint32_t *req_nq, uint64_t query_stride, uint32_t L, uint32_t k,
+                            uint64_t *tickets);
+/* nbpg_server_wait for n tickets in turn: request i's nq x k ids / dists
+ * follow request i-1's in host_ids / host_dists (either may be NULL),
+ * latency_ms[i] its latency (NULL allowed).  A failed request keeps its
+ * place and the first failure's status is returned after all n; an unknown
+ * ticket stops the call there (the later tickets stay to be waited on). */
+int nbpg_server_wait_many(nbpg_server *srv, uint32_t n, const uint64_t *tickets,
+                          uint32_t *host_ids, float *host_dists, double *latency_ms);
 int nbpg_server_stats(nbpg_server *srv, int reset, nbpg_server_stats_t *stats);
 /* finishes queued requests, lets threads already blocked in
  * nbpg_server_wait take their results, then frees the slots and the results

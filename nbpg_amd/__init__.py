@@ -111,6 +111,8 @@ _SIGS = {
     "nbpg_server_submit": (I32, [VP, VP, U32, U64, U32, U32, C.POINTER(U64)]),
     "nbpg_server_wait": (I32, [VP, U64, VP, VP, C.POINTER(DBL)]),
     "nbpg_server_stats": (I32, [VP, I32, C.POINTER(ServerStats)]),
+    "nbpg_server_submit_many": (I32, [VP, VP, U32, VP, U64, U32, U32, VP]),
+    "nbpg_server_wait_many": (I32, [VP, U32, VP, VP, VP, VP]),
     "nbpg_server_destroy": (I32, [VP]),
     "nbpg_rerank": (I32, [I32, I32, VP, U64, U32, VP, U64, VP, U32, U32, U32, U32, VP, VP, VP]),
     "nbpg_index_set_metric": (I32, [VP, I32]),
@@ -436,6 +438,36 @@ class Server:
         _check(lib().nbpg_server_wait(self.h, ticket, ids.ctypes.data, d.ctypes.data,
                                       C.byref(lat)), "nbpg_server_wait")
         return ids, d, lat.value
+
+    def submit_many(self, rows, sizes, L, k):
+        """requests of sizes[i] consecutive rows each (rows holds sum(sizes));
+        returns the tickets (uint64 array)"""
+        a = np.ascontiguousarray(rows)
+        a = a.view(np.uint8).reshape(a.shape[0], -1)
+        sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+        if int(sz.sum()) != a.shape[0]:
+            raise ValueError("sizes must add up to the rows given")
+        t = np.empty(len(sz), np.uint64)
+        _check(lib().nbpg_server_submit_many(self.h, a.ctypes.data, len(sz), sz.ctypes.data,
+                                             a.shape[1], L, k, t.ctypes.data),
+               "nbpg_server_submit_many")
+        for tk, m in zip(t.tolist(), sz.tolist()):
+            self._shape[tk] = (m, k)
+        return t
+
+    def wait_many(self, tickets):
+        """(ids, dists, latencies_ms) of the tickets' requests, rows in ticket
+        order"""
+        t = np.ascontiguousarray(tickets, dtype=np.uint64)
+        shapes = [self._shape.pop(tk) for tk in t.tolist()]
+        total = sum(m * k for m, k in shapes)
+        ids = np.empty(total, np.uint32)
+        d = np.empty(total, np.float32)
+        lat = np.empty(len(t), np.float64)
+        _check(lib().nbpg_server_wait_many(self.h, len(t), t.ctypes.data, ids.ctypes.data,
+                                           d.ctypes.data, lat.ctypes.data),
+               "nbpg_server_wait_many")
+        return ids, d, lat
 
     def stats(self, reset=False):
         st = ServerStats()

@@ -286,47 +286,77 @@ int nbpg_server_create(nbpg_server **out, const nbpg_index *ix, const nbpg_serve
 int nbpg_server_submit(nbpg_server *sv, const void *host_queries, uint32_t nq, uint64_t query_stride,
                        uint32_t L, uint32_t k, uint64_t *ticket)
 {
-    if (!sv || !host_queries || !ticket || nq == 0)
+    if (!ticket)
+        return set_error(NBPG_EINVAL, "null argument");
+    return nbpg_server_submit_many(sv, host_queries, 1, &nq, query_stride, L, k, ticket);
+}
+
+int nbpg_server_submit_many(nbpg_server *sv, const void *host_queries, uint32_t n,
+                            const uint32_t *req_nq, uint64_t query_stride, uint32_t L, uint32_t k,
+                            uint64_t *tickets)
+{
+    if (!sv || !host_queries || !tickets || !req_nq || n == 0)
         return set_error(NBPG_EINVAL, "null argument");
     if (L == 0 || L > sv->cfg.max_L || k == 0 || k > L || k > sv->cfg.max_k)
         return set_error(NBPG_EINVAL, "L = %u, k = %u (1 <= k <= L <= %u, k <= %u)", L, k,
                          sv->cfg.max_L, sv->cfg.max_k);
-    if (nq > sv->cap)
-        return set_error(NBPG_EINVAL, "request of %u queries exceeds the batch capacity %u", nq,
-                         sv->cap);
     if (query_stride < sv->ix->dims)
         return set_error(NBPG_EINVAL, "query stride too small");
-    Request *r = new Request();
-    r->nq = nq;
-    r->L = L;
-    r->k = k;
-    r->rows.assign((size_t)nq * sv->stride, 0);
-    for (uint32_t i = 0; i < nq; ++i)
-        std::memcpy(r->rows.data() + (size_t)i * sv->stride,
-                    (const uint8_t *)host_queries + i * query_stride, sv->ix->dims);
-    r->ids.resize((size_t)nq * k);
-    r->dists.resize((size_t)nq * k);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (req_nq[i] == 0)
+            return set_error(NBPG_EINVAL, "request %u has no queries", i);
+        if (req_nq[i] > sv->cap)
+            return set_error(NBPG_EINVAL, "request of %u queries exceeds the batch capacity %u",
+                             req_nq[i], sv->cap);
+    }
+    // rows copied outside the lock; all requests queued under one lock
+    std::vector<Request *> rs(n);
+    const uint8_t *src = (const uint8_t *)host_queries;
+    for (uint32_t i = 0; i < n; ++i) {
+        Request *r = new Request();
+        r->nq = req_nq[i];
+        r->L = L;
+        r->k = k;
+        r->rows.assign((size_t)r->nq * sv->stride, 0);
+        for (uint32_t j = 0; j < r->nq; ++j)
+            std::memcpy(r->rows.data() + (size_t)j * sv->stride, src + j * query_stride,
+                        sv->ix->dims);
+        src += (size_t)r->nq * query_stride;
+        r->ids.resize((size_t)r->nq * k);
+        r->dists.resize((size_t)r->nq * k);
+        rs[i] = r;
+    }
     {
         std::lock_guard<std::mutex> g(sv->mu);
         if (sv->stop) {
-            delete r;
+            for (Request *r : rs)
+                delete r;
             return set_error(NBPG_EINVAL, "server is shutting down");
         }
-        r->ticket = sv->next_ticket++;
-        r->t_submit = Clock::now();
-        sv->live[r->ticket] = r;
-        sv->pending.push_back(r);
+        const Clock::time_point now = Clock::now();
+        for (uint32_t i = 0; i < n; ++i) {
+            Request *r = rs[i];
+            r->ticket = sv->next_ticket++;
+            r->t_submit = now;
+            sv->live[r->ticket] = r;
+            sv->pending.push_back(r);
+            tickets[i] = r->ticket;
+        }
     }
     sv->cv_work.notify_one();
-    *ticket = r->ticket;
     return NBPG_OK;
 }
 
-int nbpg_server_wait(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
-                     double *latency_ms)
+} // extern "C"
+
+namespace nbpg {
+
+// nbpg_server_wait for one ticket; *entries = the request's nq * k (0 when
+// the ticket is unknown)
+static int wait_one(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
+                    double *latency_ms, size_t *entries)
 {
-    if (!sv)
-        return set_error(NBPG_EINVAL, "null argument");
+    *entries = 0;
     std::unique_lock<std::mutex> lk(sv->mu);
     auto it = sv->live.find(ticket);
     if (it == sv->live.end())
@@ -345,6 +375,7 @@ int nbpg_server_wait(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float
         sv->cv_done.notify_all(); // destroy waits for the last waiter to leave
     lk.unlock();
     int rc = r->status;
+    *entries = r->ids.size();
     if (rc == NBPG_OK) {
         if (host_ids)
             std::memcpy(host_ids, r->ids.data(), r->ids.size() * 4);
@@ -357,6 +388,45 @@ int nbpg_server_wait(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float
     }
     delete r;
     return rc;
+}
+
+} // namespace nbpg
+
+extern "C" {
+
+int nbpg_server_wait(nbpg_server *sv, uint64_t ticket, uint32_t *host_ids, float *host_dists,
+                     double *latency_ms)
+{
+    if (!sv)
+        return set_error(NBPG_EINVAL, "null argument");
+    size_t e = 0;
+    return wait_one(sv, ticket, host_ids, host_dists, latency_ms, &e);
+}
+
+int nbpg_server_wait_many(nbpg_server *sv, uint32_t n, const uint64_t *tickets, uint32_t *host_ids,
+                          float *host_dists, double *latency_ms)
+{
+    if (!sv || (n && !tickets))
+        return set_error(NBPG_EINVAL, "null argument");
+    int first = NBPG_OK;
+    std::string msg;
+    size_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        size_t e = 0;
+        const int rc = wait_one(sv, tickets[i], host_ids ? host_ids + off : nullptr,
+                                host_dists ? host_dists + off : nullptr,
+                                latency_ms ? latency_ms + i : nullptr, &e);
+        if (rc != NBPG_OK && e == 0)
+            return rc; // unknown ticket: the later requests' places are unknown
+        if (rc != NBPG_OK && first == NBPG_OK) {
+            first = rc;
+            msg = nbpg_last_error();
+        }
+        off += e;
+    }
+    if (first != NBPG_OK)
+        return set_error(first, "%s", msg.c_str());
+    return NBPG_OK;
 }
 
 int nbpg_server_stats(nbpg_server *sv, int reset, nbpg_server_stats_t *st)

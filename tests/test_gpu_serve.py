@@ -229,3 +229,44 @@ def test_served_with_seeds_equals_seeded_searches(served):
     assert ra >= rb - 0.01, (ra, rb)
     with pytest.raises(N.NbpgError):
         N.Server(ix, fast=FAST, max_L=300, start_points=seeds)
+
+
+def test_burst_submit_and_wait(served):
+    """nbpg_server_submit_many / wait_many: requests queued in bursts return,
+    request by request, what each returns alone; bad bursts are refused
+    whole, and an unknown ticket stops wait_many with the later tickets still
+    waitable one by one."""
+    N, ix, reqs = served
+    L, k = 48, 10
+    rows = np.concatenate([r[0] for r in reqs])
+    sizes = np.array([len(r[0]) for r in reqs], np.uint32)
+    alone = run_alone(N, ix, [(r[0], L, k) for r in reqs])
+    srv = N.Server(ix, slots=2, max_batch_queries=256, fast=FAST)
+    off = 0
+    got = []
+    for lo in range(0, len(sizes), 7):  # bursts of 7 requests, the last shorter
+        sz = sizes[lo:lo + 7]
+        m = int(sz.sum())
+        t = srv.submit_many(rows[off:off + m], sz, L, k)
+        ids, d, lat = srv.wait_many(t)
+        assert (lat > 0).all()
+        got.append((ids.reshape(m, k), d.reshape(m, k)))
+        off += m
+    ids = np.concatenate([g[0] for g in got])
+    d = np.concatenate([g[1] for g in got])
+    assert (ids == np.concatenate([a[0] for a in alone])).all()
+    assert (d == np.concatenate([a[1] for a in alone])).all()
+    with pytest.raises(N.NbpgError):  # one request over the batch capacity refuses all
+        srv.submit_many(rows[:300], np.array([10, 290], np.uint32), L, k)
+    with pytest.raises(N.NbpgError):  # an empty request
+        srv.submit_many(rows[:10], np.array([10, 0], np.uint32), L, k)
+    t = srv.submit_many(rows[:30], np.array([10, 20], np.uint32), L, k)
+    srv._shape[12345678] = (1, k)
+    with pytest.raises(N.NbpgError):
+        srv.wait_many(np.array([12345678, t[0], t[1]], np.uint64))
+    for tk, want in zip(t.tolist(), (alone[0], None)):
+        srv._shape[tk] = (10 if want is not None else 20, k)
+        ids1, _, _ = srv.wait(tk)
+        if want is not None:
+            assert (ids1 == np.concatenate([a[0] for a in alone])[:10]).all()
+    srv.close()

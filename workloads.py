@@ -687,8 +687,8 @@ def run_serve(args, log):
                                    for t, v in picks.items()}}
     srv = N.Server(ix, slots=3, max_batch_queries=4096, mem_budget=256 << 20, max_L=256,
                    max_k=10, fast=fast, start_points=sp_use)
-    loads = serve_loads(srv, q, gt, gtd, L_pick, [0.01 * full_qps, 0.04 * full_qps, None], 4.0,
-                        log)
+    loads = serve_loads(srv, q, gt, gtd, L_pick,
+                        [0.01 * full_qps, 0.04 * full_qps, None, ("burst", 64)], 4.0, log)
     srv.close()
     res["loads"] = loads
     ix.close()
@@ -698,11 +698,13 @@ def run_serve(args, log):
 def serve_loads(srv, q, gt, gtd, L, targets, duration, log, k=10, seed=5):
     """Drive a running nbpg server with requests of 1..64 consecutive rows
     of q (k results each, beam width L) for `duration` seconds per target:
-    Poisson arrivals at each offered QPS in `targets` (queries per second),
-    or, for a target of None, a closed loop with 256 requests outstanding.
-    One row per target: offered / submitted / served QPS, the server's
-    p50 / p99 / max request latency (submit to completion), mean batch size
-    and recall@k of the served results against (gt, gtd)."""
+    Poisson arrivals at each offered QPS in `targets` (queries per second);
+    None: a closed loop with 256 single requests outstanding; ("burst", B):
+    a closed loop that submits bursts of B requests in one call
+    (nbpg_server_submit_many) and keeps 8 bursts outstanding.  One row per
+    target: offered / submitted / served QPS, the server's p50 / p99 / max
+    request latency (submit to completion), mean batch size and recall@k of
+    the first 100k served queries against (gt, gtd)."""
     rng = np.random.default_rng(seed)
     nq = q.shape[0]
     sizes = rng.integers(1, 65, 400_000)
@@ -710,24 +712,31 @@ def serve_loads(srv, q, gt, gtd, L, targets, duration, log, k=10, seed=5):
     loads = []
     for target in targets:
         srv.stats(reset=True)
-        gaps = rng.exponential(32.5 / (target or 1.0), 400_000)
+        burst = target[1] if isinstance(target, tuple) else 0
+        gaps = rng.exponential(32.5 / (target if isinstance(target, float) else 1.0), 400_000)
         tickets = []
         done = threading.Event()
         results = {}
-        window = threading.Semaphore(256) if target is None else None
+        window = threading.Semaphore(8 if burst else 256) if not isinstance(target, float) \
+            else None
 
         def consumer():
-            i = 0
+            i, kept = 0, 0
             while True:
                 while i >= len(tickets):
                     if done.is_set() and i >= len(tickets):
                         return
                     time.sleep(0.0002)
                 tk, s0, m = tickets[i]
-                ids, _, _ = srv.wait(tk)
+                if burst:
+                    ids = srv.wait_many(tk)[0].reshape(m, k)
+                else:
+                    ids = srv.wait(tk)[0]
                 if window is not None:
                     window.release()
-                results[i] = (s0, m, ids)
+                # ids kept for the recall of the first 100k served queries
+                results[i] = (s0, m, ids if kept < 100_000 else None)
+                kept += m
                 i += 1
 
         th = threading.Thread(target=consumer, daemon=True)
@@ -750,8 +759,14 @@ def serve_loads(srv, q, gt, gtd, L, targets, duration, log, k=10, seed=5):
                         time.sleep(max(0.0, nxt - now - 0.0002))
                         while time.perf_counter() < nxt:
                             pass
-                s0, m = int(starts[i]), int(sizes[i])
-                tickets.append((srv.submit(q[s0:s0 + m], L, k), s0, m))
+                if burst:
+                    sz = sizes[(i * burst) % (len(sizes) - burst):][:burst]
+                    m = int(sz.sum())
+                    s0 = int(starts[i]) % (nq - m + 1)
+                    tickets.append((srv.submit_many(q[s0:s0 + m], sz, L, k), s0, m))
+                else:
+                    s0, m = int(starts[i]), int(sizes[i])
+                    tickets.append((srv.submit(q[s0:s0 + m], L, k), s0, m))
                 i += 1
         finally:
             submit_end = time.perf_counter()
@@ -759,12 +774,19 @@ def serve_loads(srv, q, gt, gtd, L, targets, duration, log, k=10, seed=5):
         th.join(timeout=60)
         wall = time.perf_counter() - t0
         st = srv.stats()
-        got = [j for j in range(len(tickets)) if j in results]
+        # recall over the first 100k served queries (all of them in the open loops)
+        got = [j for j in range(len(tickets)) if j in results and results[j][2] is not None]
         ids = np.concatenate([results[j][2] for j in got])
         qi = np.concatenate([np.arange(results[j][0], results[j][0] + results[j][1]) for j in got])
         rec = recall(ids, gt[qi], gtd[qi], k)
-        row = {"offered_qps": round(target, 1) if target else "closed loop, 256 requests "
-               "outstanding", "submitted_qps": round(st["queries"] / (submit_end - t0), 1),
+        if isinstance(target, float):
+            offered = round(target, 1)
+        elif burst:
+            offered = "closed loop, bursts of %d requests, 8 bursts outstanding" % burst
+        else:
+            offered = "closed loop, 256 requests outstanding"
+        row = {"offered_qps": offered,
+               "submitted_qps": round(st["queries"] / (submit_end - t0), 1),
                "served_qps": round(st["queries"] / wall, 1), "requests": st["requests"],
                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                "max_ms": round(st["max_ms"], 3),
