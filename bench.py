@@ -56,6 +56,13 @@ def main():
                          "autotuner's choice, or the best of 1..4)")
     ap.add_argument("--no-entry0", action="store_true",
                     help="skip timing the FAST path from the single entry node 0 (profiling runs)")
+    ap.add_argument("--start", choices=["kmeans", "nearest"], default="kmeans",
+                    help="FAST start points: kmeans = --seeds k-means seeds shared by every "
+                         "query; nearest = each query's nearest of --seed-rows sampled rows "
+                         "(NBPG_SEARCH_SEED_NEAREST, found by an int8-MFMA pass before the "
+                         "search).  The other rule is timed beside it")
+    ap.add_argument("--seed-rows", type=int, default=1024,
+                    help="seed rows for the nearest-seed starts (at most 4096)")
     ap.add_argument("--seeds", type=int, default=64,
                     help="FAST-mode start nodes from k-means over the base (1 = node 0 only)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -183,19 +190,45 @@ def main():
             rs.append(AT.recall_with_ties(ids.cpu().numpy().view(np.uint32), gts[b], gtds[b], 10))
         return float(np.mean(rs))
 
-    # FAST-mode start nodes: 64 k-means seeds over the base (the plain path
-    # keeps the reference's single entry, node 0)
-    seeds = None
-    if args.seeds > 1:
+    # FAST-mode start points (the plain path keeps the reference's single
+    # entry, node 0): each query's nearest of 4096 sampled seed rows (the
+    # per-query get_sp of the reference's driver, chosen on the device before
+    # the search), or k-means seeds shared by every query
+    seeds = None  # start points shared by every query
+    fmode = N.SEARCH_FAST
+    srows = np.sort(np.random.default_rng(123).choice(
+        base.shape[0], min(args.seed_rows, N.MAX_SEEDS, base.shape[0]),
+        replace=False)).astype(np.uint32)
+    near_desc = ("each query's nearest of %d sampled seed rows (int8-MFMA pass before the "
+                 "search)" % len(srows))
+    # the other start rule, timed beside the headline's: (mode, shared
+    # start points, description, JSON key)
+    alt = None
+    if args.start == "nearest":
+        ix.set_seeds(srows)
+        fmode |= N.SEARCH_SEED_NEAREST
+        start_desc = near_desc
+        if not args.no_entry0 and args.seeds > 1:
+            ks = N.pick_seeds(ix, args.seeds)
+            alt = (N.SEARCH_FAST, ks, "%d k-means seeds" % len(ks), "fast_kmeans_seeds")
+    elif args.seeds > 1:
         t = time.time()
         seeds = N.pick_seeds(ix, args.seeds)
         log("%d k-means seeds in %.2fs" % (len(seeds), time.time() - t))
+        start_desc = "%d k-means seeds" % len(seeds)
+        if not args.no_entry0:
+            ix.set_seeds(srows)
+            alt = (N.SEARCH_FAST | N.SEARCH_SEED_NEAREST, None, near_desc,
+                   "fast_nearest_seed_rows")
+    else:
+        start_desc = "node 0"
+    log("FAST start points:", start_desc)
     tune_table = None
     streams = args.streams
     if args.fast == "auto":
         t = time.time()
         best, tune_table = AT.tune(ix, q_d, gt, gtd, k=10, target=args.target, log=log,
-                                   start_points=seeds, q_batches=q_ds)
+                                   start_points=seeds, q_batches=q_ds, mode=fmode)
         fast = best["config"]
         log("autotuner picked %s at L=%d, %d streams in %.1fs"
             % (fast, best["L"], best["streams"], time.time() - t))
@@ -208,8 +241,12 @@ def main():
     full_sweep = [10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64, 80, 96, 128, 160, 200, 256]
 
     def run_fast(L, out=None, batch=0):
-        return N.search(ix, q_ds[batch], L, 10, mode=N.SEARCH_FAST, fast=fast, out=out,
+        return N.search(ix, q_ds[batch], L, 10, mode=fmode, fast=fast, out=out,
                         start_points=seeds)
+
+    def run_fast_alt(L, out=None, batch=0):
+        return N.search(ix, q_ds[batch], L, 10, mode=alt[0], fast=fast, out=out,
+                        start_points=alt[1])
 
     def run_fast_entry0(L, out=None, batch=0):
         return N.search(ix, q_ds[batch], L, 10, mode=N.SEARCH_FAST, fast=fast, out=out)
@@ -228,8 +265,9 @@ def main():
 
     L_fast, rec_fast = pick(run_fast, "fast", args.L)
     L_plain, rec_plain = pick(run_plain, "plain")
-    cmp_e0 = seeds is not None and not args.no_entry0
+    cmp_e0 = (seeds is not None or args.start == "nearest") and not args.no_entry0
     L_e0, rec_e0 = pick(run_fast_entry0, "fast, entry node 0") if cmp_e0 else (L_fast, rec_fast)
+    L_alt, rec_alt = pick(run_fast_alt, "fast, " + alt[2]) if alt is not None else (None, None)
 
     def launchers(runner, L, S):
         """launch(i) searches batch i % B into its own output set (step i
@@ -238,12 +276,15 @@ def main():
         outs = [runner(L, batch=j % B) for j in range(M)]
         return (lambda i: runner(L, outs[i % M], batch=i % B)), outs
 
-    def graph_launchers(L, S, entry0=False):
+    def graph_launchers(L, S, starts="main"):
         # each batch's search captured by the library as a HIP graph
-        # (nbpg_search_graph_*), one graph per output set
+        # (nbpg_search_graph_*), one graph per output set; starts: the
+        # headline's, node 0 ("entry0") or the other start rule ("alt")
         M = AT.output_sets(B, S)
-        gs = [N.SearchGraph(ix, q_ds[j % B], L, 10, mode=N.SEARCH_FAST, fast=fast,
-                            start_points=None if entry0 else seeds) for j in range(M)]
+        m = {"main": fmode, "entry0": N.SEARCH_FAST, "alt": alt[0] if alt else None}[starts]
+        sp = {"main": seeds, "entry0": None, "alt": alt[1] if alt else None}[starts]
+        gs = [N.SearchGraph(ix, q_ds[j % B], L, 10, mode=m, fast=fast, start_points=sp)
+              for j in range(M)]
         return (lambda i: gs[i % M].replay()), gs
 
     def pipelined(launch, S, steps, warmup, per_batch=False):
@@ -330,7 +371,7 @@ def main():
 
     def e2e_step():
         q_e2e.copy_(qh, non_blocking=True)
-        N.search(ix, q_e2e, L_fast, 10, mode=N.SEARCH_FAST, fast=fast, out=out_e2e,
+        N.search(ix, q_e2e, L_fast, 10, mode=fmode, fast=fast, out=out_e2e,
                  start_points=seeds)
         ids_h.copy_(out_e2e[0], non_blocking=True)
         d_h.copy_(out_e2e[1], non_blocking=True)
@@ -350,11 +391,22 @@ def main():
     e0_qps = qps
     if cmp_e0:
         K0 = max(6, K // 2)
-        l0, _ = (graph_launchers(L_e0, S, entry0=True) if use_graph
+        l0, _ = (graph_launchers(L_e0, S, starts="entry0") if use_graph
                  else launchers(run_fast_entry0, L_e0, S))
         w0, _, _ = pipelined(l0, S, K0, 2)
         e0_qps = nq * K0 / w0
         log("fast with the single entry node 0, L=%d: %.0f QPS" % (L_e0, e0_qps))
+    alt_qps, alt_alone = None, None
+    if alt is not None:
+        la, _ = (graph_launchers(L_alt, S, starts="alt") if use_graph
+                 else launchers(run_fast_alt, L_alt, S))
+        wa, _, _ = pipelined(la, S, K, args.warmup)
+        alt_qps = nq * K / wa
+        la1, _ = launchers(run_fast_alt, L_alt, 1)
+        _, _, per_a = pipelined(la1, 1, K, args.warmup, per_batch=True)
+        alt_alone = float(np.percentile(per_a, 50))
+        log("fast from %s, L=%d: %.0f QPS (%d streams), one batch alone p50 %.3f ms"
+            % (alt[2], L_alt, alt_qps, S, alt_alone))
     lp, _ = launchers(run_plain, L_plain, S)
     pwall, _, _ = pipelined(lp, S, args.plain_steps, 1)
     plain_qps = nq * args.plain_steps / pwall
@@ -370,14 +422,13 @@ def main():
         import workloads as WL  # noqa: E402  (bench-only module beside this file)
         srv = N.Server(ix, slots=3, max_batch_queries=4096, mem_budget=256 << 20,
                        max_L=max(64, L_fast), max_k=10, fast=tuple(fast[:3]) + (0,) + tuple(fast[4:]),
-                       start_points=seeds)
+                       start_points=seeds, seed_nearest=args.start == "nearest")
         loads = WL.serve_loads(srv, q, gt, gtd, L_fast,
                                [0.0025 * qps, 0.01 * qps, None, ("burst", 64)],
                                args.serve_seconds, log)
         srv.close()
         served = {"L": L_fast, "k": 10, "slots": 3, "request_queries": "1..64",
-                  "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
-                  else "node 0", "loads": loads,
+                  "start_nodes": start_desc, "loads": loads,
                   "what": "the server on this index: Poisson arrivals at 0.25% and 1% of the "
                           "batched value, a closed loop of single requests (bounded by the "
                           "Python client's submit rate), and a closed loop submitting bursts "
@@ -387,15 +438,16 @@ def main():
     # query's own counters -- expansions, distance computations and the
     # adjacency entries its expanded nodes hold (NBPG_SEARCH_EDGES):
     #   B_q = 4 * (visited + edges) + D * dist_cmps + D
-    # averaged over the batches.  The seed rows' distances (every query
-    # starts with one per seed) hit the same L2-resident rows for every
-    # query; the headline figure leaves them out and the figure with them is
-    # reported beside it.
+    # averaged over the batches.  The start rows' distances (one per shared
+    # seed, or the query's nearest seed row) read rows every query's search
+    # shares, from L2; the headline figure leaves them out and the figure
+    # with them is reported beside it.  The nearest-seed pass before the
+    # search reads its seed rows from L2 too (its time is in the step).
     dbytes = base.shape[1]
     nseed = len(seeds) if seeds is not None else 1
     st_all = []
     for b in range(B):
-        e_ids, _, st3 = N.search(ix, q_ds[b], L_fast, 10, mode=N.SEARCH_FAST, fast=fast,
+        e_ids, _, st3 = N.search(ix, q_ds[b], L_fast, 10, mode=fmode, fast=fast,
                                  start_points=seeds, edges=True)
         if not torch.equal(e_ids, run_fast(L_fast, batch=b)[0]):
             raise SystemExit("FAST results differ between two calls")
@@ -416,7 +468,8 @@ def main():
                                                  achieved_seeds, alone_ms,
                                                  alg_bytes / (alone_ms * 1e-3) / 1e9))
 
-    traffic, traffic_cfg = measured_traffic(base.shape[0], nq, base.shape[1], L_fast, fast)
+    traffic, traffic_cfg = measured_traffic(base.shape[0], nq, base.shape[1], L_fast, fast,
+                                            "nearest" if args.start == "nearest" else "kmeans")
     cpu = None
     if not args.no_cpu:
         global GTD
@@ -441,8 +494,7 @@ def main():
                                "device, k=10, fast persistent search", "n": base.shape[0],
                    "nq": nq, "query_batches": B, "L": L_fast, "fast_config": list(fast),
                    "recall@10": round(rec_fast, 4),
-                   "start_nodes": ("%d k-means seeds" % len(seeds)) if seeds is not None
-                   else "node 0",
+                   "start_nodes": start_desc,
                    "streams": S, "launch": "hipgraph" if use_graph else "eager",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -479,6 +531,10 @@ def main():
         "served": served,
         "plain": {"L": L_plain, "qps": round(plain_qps, 1), "recall@10": round(rec_plain, 4),
                   "streams": S, "speedup_fast_over_plain": round(qps / plain_qps, 2)},
+        (alt[3] if alt else "fast_other_start"): {
+            "start_nodes": alt[2], "L": L_alt, "qps": round(alt_qps, 1),
+            "recall@10": round(rec_alt, 4), "alone_p50_ms": round(alt_alone, 4)}
+        if alt_qps is not None else None,
         "fast_entry_node0": {"L": L_e0, "qps": round(e0_qps, 1), "recall@10": round(rec_e0, 4)}
         if cmp_e0 else None,
         "build_s": round(build_s, 2),
@@ -499,21 +555,24 @@ def main():
 GTD = None
 
 
-def measured_traffic(n, nq, dims, L, fast):
+def measured_traffic(n, nq, dims, L, fast, start):
     """HBM bytes per FAST launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries of this workload (calibrated against a 1 GiB copy),
     one per profiled launch shape: (bytes, profiled shape).  A shape that
     differs only in its group count takes the bytes of the profiled one (the
     grid size changes occupancy, not what a query reads: 2048 and 3072
-    groups fetched within 0.3% of each other).  (None, None) when this
-    workload or kernel shape was not profiled."""
+    groups fetched within 0.3% of each other).  start: "nearest" (each
+    query's nearest seed row) or "kmeans" (64 shared k-means seeds); the
+    profiled run must have used the same.  (None, None) when this workload,
+    start rule or kernel shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "search_reg_kernel_1m_u8_L16.json")
     try:
         with open(path) as f:
             prof = json.load(f)
     except OSError:
         return None, None
-    same = [p for p in prof["runs"] if (p["n"], p["nq"], p["dims"], p["L"]) == (n, nq, dims, L)]
+    same = [p for p in prof["runs"] if (p["n"], p["nq"], p["dims"], p["L"]) == (n, nq, dims, L)
+            and p.get("start", "").startswith(start if start == "nearest" else "64 k-means")]
     for p in same:
         if list(p["fast_config"]) == list(fast):
             return p["traffic_bytes_per_launch"], list(fast)

@@ -67,6 +67,18 @@ extern "C" {
  * NBPG_VISITED_CAP(L) ids per query (output stride), 0xFFFFFFFF past the
  * count; d_out_dists is not written.  L <= 256. */
 #define NBPG_SEARCH_VISITED 0x40
+/* OR into nbpg_search's NBPG_SEARCH_FAST mode: every query starts from the
+ * nearest of the index's seed rows (nbpg_index_set_seeds) -- the per-query
+ * start points the reference's driver asks get_sp(i) for
+ * (beamSearch.h:257-303, check_nn_recall.h:282-297) -- instead of the host
+ * start points (still checked, not used).  A pass before the search finds
+ * each query's nearest seed from exact integer distances on int8 matrix
+ * cores (ties to the lower id) and leaves its id in the query's first
+ * d_out_ids entry, where the search reads it before writing its results
+ * (so d_out_ids is required).  u8 / i8 rows of a multiple of 16 bytes,
+ * 16-byte aligned query rows, L <= 256; not with NBPG_SEARCH_VISITED. */
+#define NBPG_SEARCH_SEED_NEAREST 0x80
+#define NBPG_MAX_SEEDS 4096
 /* visited-list capacity per query: the reference's 8L (beamSearch.h:279),
  * at most 1024 */
 #define NBPG_VISITED_CAP(L) ((8u * (L)) < 1024u ? (8u * (L)) : 1024u)
@@ -150,6 +162,12 @@ int nbpg_index_destroy(nbpg_index *idx);
 /* the index's distance (NBPG_METRIC_L2 at creation): every search, prune,
  * build and distance call on it uses this metric afterwards */
 int nbpg_index_set_metric(nbpg_index *idx, int metric);
+/* the index's seed rows for NBPG_SEARCH_SEED_NEAREST: count (1..NBPG_MAX_SEEDS)
+ * distinct host ids < n, kept in ascending order with a copy of their rows
+ * taken now (count 0 drops them).  nbpg_index_set_points drops them too:
+ * set them again after new rows.  No counterpart in the reference (its
+ * callers pass start points per query through get_sp). */
+int nbpg_index_set_seeds(nbpg_index *idx, const uint32_t *h_ids, uint32_t count);
 /* upload rows [first, first+count) from host memory with the given stride
  * (the reference's 64-B padded rows, point_range.h:51-56, or packed) */
 int nbpg_index_set_points(nbpg_index *idx, const void *host_rows, uint64_t host_stride,
@@ -363,6 +381,8 @@ typedef struct {
     nbpg_fast_config fast;      /* launch shape of every batch */
     uint32_t sp_num;            /* start points: 1..64 distinct ids (more than one: max_L <= 256) */
     uint32_t start_points[64];  /* e.g. from nbpg_pick_seeds */
+    uint32_t seed_nearest;      /* 1: each query starts from its nearest seed row of the index
+                                   (NBPG_SEARCH_SEED_NEAREST; max_L <= 256), start points unused */
 } nbpg_server_config;
 
 typedef struct {

@@ -21,6 +21,8 @@ CODE_F32, CODE_U8, CODE_I8, CODE_U4, CODE_I4 = 0, 1, 2, 3, 4
 CODE_OF_EXT = {".fbin": CODE_F32, ".u8bin": CODE_U8, ".i8bin": CODE_I8, ".u4bin": CODE_U4,
                ".i4bin": CODE_I4}
 SEARCH_REF, SEARCH_FAST, SEARCH_FRONTIER, SEARCH_EDGES, SEARCH_VISITED = 0, 1, 0x10, 0x20, 0x40
+SEARCH_SEED_NEAREST = 0x80  # FAST: each query starts from its nearest seed row (set_seeds)
+MAX_SEEDS = 4096
 
 
 def visited_cap(L):
@@ -64,7 +66,7 @@ U32, U64, I32, DBL = C.c_uint32, C.c_uint64, C.c_int, C.c_double
 class ServerConfig(C.Structure):
     _fields_ = [("slots", U32), ("max_batch_queries", U32), ("mem_budget", U64),
                 ("max_L", U32), ("max_k", U32), ("fast", FastConfig), ("sp_num", U32),
-                ("start_points", U32 * 64)]
+                ("start_points", U32 * 64), ("seed_nearest", U32)]
 
 
 class ServerStats(C.Structure):
@@ -116,6 +118,7 @@ _SIGS = {
     "nbpg_server_destroy": (I32, [VP]),
     "nbpg_rerank": (I32, [I32, I32, VP, U64, U32, VP, U64, VP, U32, U32, U32, U32, VP, VP, VP]),
     "nbpg_index_set_metric": (I32, [VP, I32]),
+    "nbpg_index_set_seeds": (I32, [VP, VP, U32]),
     "nbpg_pca_project": (I32, [I32, VP, U64, U32, VP, VP, U32, I32, I32, VP, VP, VP]),
     "nbpg_pca_quantize": (I32, [VP, U64, U32, I32, VP, VP]),
     "nbpg_pca_stats_f64": (I32, [I32, VP, U64, U32, VP, VP, VP]),
@@ -193,6 +196,12 @@ class Index:
         assert a.ndim == 2 and a.shape[1] * a.itemsize >= self.dims
         _check(lib().nbpg_index_set_points(self.h, a.ctypes.data, a.shape[1] * a.itemsize, first,
                                            a.shape[0]), "nbpg_index_set_points")
+
+    def set_seeds(self, ids):
+        """seed rows for SEARCH_SEED_NEAREST (distinct ids < n, at most
+        MAX_SEEDS; an empty list drops them)"""
+        a = np.ascontiguousarray(ids, dtype=np.uint32)
+        _check(lib().nbpg_index_set_seeds(self.h, a.ctypes.data, len(a)), "nbpg_index_set_seeds")
 
     def set_graph(self, rows, first=0):
         a = np.ascontiguousarray(rows, dtype=np.uint32)
@@ -408,14 +417,15 @@ class Server:
     wait() releases the GIL while it blocks."""
 
     def __init__(self, index, slots=3, max_batch_queries=4096, mem_budget=0, max_L=256,
-                 max_k=100, entry=0, fast=(2, 1, 11, 0, 0), start_points=None):
+                 max_k=100, entry=0, fast=(2, 1, 11, 0, 0), start_points=None,
+                 seed_nearest=False):
         _torch()
         self.index = index
         sp = [entry] if start_points is None else [int(x) for x in start_points]
         if not 1 <= len(sp) <= 64:
             raise ValueError("1..64 start points")
         cfg = ServerConfig(slots, max_batch_queries, mem_budget, max_L, max_k, FastConfig(*fast),
-                           len(sp), (U32 * 64)(*sp))
+                           len(sp), (U32 * 64)(*sp), int(bool(seed_nearest)))
         h = VP()
         _check(lib().nbpg_server_create(C.byref(h), index.h, C.byref(cfg)), "nbpg_server_create")
         self.h = h

@@ -76,11 +76,13 @@ def output_sets(batches, streams):
 
 
 def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAULT_L, steps=24,
-         log=None, start_points=None, q_batches=None):
+         log=None, start_points=None, q_batches=None, mode=None):
     """Returns (best, table): best = dict(config, L, streams, ms, qps,
     recall); ms is per batch.  L is chosen on q_dev (with its ground truth);
     timing searches the batches of q_batches in turn (default: q_dev
-    alone), each batch the size of q_dev."""
+    alone), each batch the size of q_dev.  mode: the FAST mode bits
+    (default N.SEARCH_FAST; e.g. | N.SEARCH_SEED_NEAREST)."""
+    mode = N.SEARCH_FAST if mode is None else mode
     import torch
     space = space or DEFAULT_SPACE
     nq = q_dev.shape[0]
@@ -96,7 +98,7 @@ def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAU
             for L in Ls:
                 if L < k:
                     continue
-                ids, _, _ = N.search(index, q_dev, L, k, mode=N.SEARCH_FAST, fast=cfg, stats=False,
+                ids, _, _ = N.search(index, q_dev, L, k, mode=mode, fast=cfg, stats=False,
                                      start_points=start_points)
                 torch.cuda.synchronize()
                 r = recall_with_ties(ids.cpu().numpy().view(np.uint32), gt_ids, gt_dists, k)
@@ -110,11 +112,11 @@ def tune(index, q_dev, gt_ids, gt_dists, k=10, target=0.95, space=None, Ls=DEFAU
             rows = []
             for st in space["streams"]:
                 M = output_sets(len(qb), st)
-                outs = [N.search(index, qb[j % len(qb)], L, k, mode=N.SEARCH_FAST, fast=cfg,
+                outs = [N.search(index, qb[j % len(qb)], L, k, mode=mode, fast=cfg,
                                  stats=False, start_points=start_points) for j in range(M)]
 
                 def launch(i):
-                    N.search(index, qb[i % len(qb)], L, k, mode=N.SEARCH_FAST, fast=cfg,
+                    N.search(index, qb[i % len(qb)], L, k, mode=mode, fast=cfg,
                              out=outs[i % M], start_points=start_points)
                 ms = pipelined_ms(launch, st, steps)
                 rows.append({"config": cfg, "L": L, "streams": st, "recall": r, "ms": ms,

@@ -313,6 +313,7 @@ int nbpg_index_destroy(nbpg_index *ix)
     if (!ix)
         return NBPG_OK;
     hipDeviceSynchronize();
+    drop_seeds(ix);
     hipFree(ix->vecs);
     hipFree(ix->norms);
     hipFree(ix->nbr);
@@ -352,6 +353,7 @@ int nbpg_index_set_points(nbpg_index *ix, const void *host_rows, uint64_t host_s
                          (unsigned long long)ix->n, (unsigned long long)host_stride);
     if (count == 0)
         return NBPG_OK;
+    drop_seeds(ix); // their copies would no longer match the rows
     // a flat copy only when a host row has no bytes past `dims`: padding
     // bytes (the reference leaves them uninitialised, point_range.h:51-56)
     // must not reach the device rows, whose tails every kernel sums as zeros

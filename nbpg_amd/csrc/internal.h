@@ -38,6 +38,14 @@ struct nbpg_index {
     // exit count)
     uint32_t *counter;
     uint32_t launch_seq; // next work counter (host side, atomic)
+    // seed rows for nearest-seed FAST starts (start.hip): n_seeds ids
+    // ascending, their rows as signed bytes (stride each, zero past dims) and
+    // their seed terms for squared L2 and the inner product, padded to whole
+    // 16-row tiles
+    uint32_t n_seeds;
+    uint32_t *seed_ids;
+    uint8_t *seed_rows;
+    int32_t *seed_self_l2, *seed_self_ip;
 };
 
 namespace nbpg {
@@ -68,6 +76,12 @@ struct DevInfo {
 extern DevInfo g_dev;
 
 int ensure_init();
+// nearest-seed FAST starts (start.hip): the index's seed rows, and the pass
+// that writes each query's start id to d_out_ids[q * out_k]
+bool start_select_supported(const nbpg_index *ix);
+int start_select(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
+                 uint32_t *d_out_ids, uint32_t out_k, hipStream_t stream);
+void drop_seeds(nbpg_index *ix);
 int ensure_ws(nbpg_index *idx, size_t bytes);
 // A persistent search's work counter (zero when idle; the launch's last wave
 // resets it).  Launches take the index's counters in turn, so launches on

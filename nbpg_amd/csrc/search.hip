@@ -400,7 +400,7 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                 const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
                 uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
                 uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w,
-                uint32_t *vis_out, uint32_t vis_cap, bool vis_cmps)
+                uint32_t *vis_out, uint32_t vis_cap, bool vis_cmps, bool qstart)
 {
     nbpg_fast_config c = {2, 1, 11, 0, 0};
     if (cfg)
@@ -489,6 +489,12 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
     dim3 g(groups), b(64 * c.waves_per_group);
     if (nseeds > 1 && !reg)
         return set_error(NBPG_EUNSUP, "seeded FAST search needs L <= 256");
+    if (qstart) {
+        if (!reg || vis_out || !d_ids)
+            return set_error(NBPG_EUNSUP, "nearest-seed starts need L <= 256 and the id output");
+        a.qstart = 1;
+        a.nseeds = 0;
+    }
     if (reg)
         return launch_search_reg(ix->dtype, ix->nch, ipl, E, split, g, b, (uint32_t)lds, stream, a);
     int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
@@ -580,9 +586,14 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
     const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
     const bool visited = (mode & NBPG_SEARCH_VISITED) != 0;
-    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES | NBPG_SEARCH_VISITED);
-    if ((edges || visited) && base != NBPG_SEARCH_FAST)
-        return set_error(NBPG_EINVAL, "NBPG_SEARCH_EDGES / NBPG_SEARCH_VISITED are FAST-mode outputs");
+    const bool nearest = (mode & NBPG_SEARCH_SEED_NEAREST) != 0;
+    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES | NBPG_SEARCH_VISITED |
+                              NBPG_SEARCH_SEED_NEAREST);
+    if ((edges || visited || nearest) && base != NBPG_SEARCH_FAST)
+        return set_error(NBPG_EINVAL, "NBPG_SEARCH_EDGES / VISITED / SEED_NEAREST are FAST-mode "
+                                      "options");
+    if (nearest && visited)
+        return set_error(NBPG_EINVAL, "NBPG_SEARCH_SEED_NEAREST needs the id output");
     if (visited && frontier)
         return set_error(NBPG_EINVAL, "NBPG_SEARCH_VISITED replaces the frontier output");
     NBPG_TRY(check_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, base,
@@ -601,10 +612,18 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
     if (base == NBPG_SEARCH_REF)
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, out_k,
                           nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
+    if (nearest) {
+        if (out_k == 0)
+            return set_error(NBPG_EINVAL, "NBPG_SEARCH_SEED_NEAREST needs k >= 1");
+        if (qp->L > 4 * WAVE)
+            return set_error(NBPG_EUNSUP, "nearest-seed starts need L <= 256");
+        NBPG_TRY(start_select(ix, d_queries, nq, query_stride, d_out_ids, out_k,
+                              as_stream(stream)));
+    }
     return search_fast(ix, d_queries, nq, query_stride, h_start_points[0], h_start_points, sp_num,
                        qp->L, out_k, nullptr, nullptr, qp->L, out_k, qp->limit, fast, d_out_ids,
                        d_out_dists, d_out_stats, counter ? counter : work_counter(ix),
-                       as_stream(stream), edges ? 3 : 2);
+                       as_stream(stream), edges ? 3 : 2, nullptr, 0, false, nearest);
 }
 
 int nbpg_search(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t query_stride,

@@ -31,6 +31,7 @@ struct SearchArgs {
     const uint32_t *Ls, *ks;
     uint32_t out_k, Lmax, W, flog2, entry, wave_bytes;
     uint32_t nseeds; // FAST: > 1 seeds in sp[] replace the single entry
+    uint32_t qstart; // FAST: 1 = query q starts from out_ids[q * out_k] (nearest seed)
     uint32_t *counter;
     // outputs
     uint32_t *out_ids;
@@ -97,7 +98,8 @@ int search_fast(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64
                 const uint32_t *d_L, const uint32_t *d_k, uint32_t Lmax, uint32_t out_k,
                 uint64_t limit, const nbpg_fast_config *cfg, uint32_t *d_ids, float *d_dists,
                 uint32_t *d_stats, uint32_t *counter, hipStream_t stream, uint32_t stats_w = 2,
-                uint32_t *vis_out = nullptr, uint32_t vis_cap = 0, bool vis_cmps = false);
+                uint32_t *vis_out = nullptr, uint32_t vis_cap = 0, bool vis_cmps = false,
+                bool qstart = false);
 
 int launch_search_reg(int dtype, uint32_t nch, uint32_t ipl, uint32_t E, bool split, dim3 grid,
                       dim3 block, uint32_t lds, hipStream_t stream, const SearchArgs &a);

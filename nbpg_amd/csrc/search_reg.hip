@@ -131,7 +131,8 @@ __global__ __launch_bounds__(1024) void search_reg_kernel(SearchArgs a)
         // call for both cases keeps the query reads in one place.
         const uint32_t ns = a.nseeds > 1 ? a.nseeds : 1u;
         {
-            uint32_t sid = lane == 0 ? a.entry : NBPG_SENT;
+            uint32_t sid = lane == 0 ? (a.qstart ? a.out_ids[(uint64_t)qi * a.out_k] : a.entry)
+                                     : NBPG_SENT;
             if (a.nseeds > 1)
                 for (uint32_t j = 0; j < a.nseeds; ++j)
                     sid = lane == j ? a.sp[j] : sid;
@@ -413,7 +414,8 @@ __global__ __launch_bounds__(1024) void search_split_kernel(SearchArgs a)
         wave_sync();
         const uint32_t ns = a.nseeds > 1 ? a.nseeds : 1u;
         {
-            uint32_t sid = lane == 0 ? a.entry : NBPG_SENT;
+            uint32_t sid = lane == 0 ? (a.qstart ? a.out_ids[(uint64_t)qi * a.out_k] : a.entry)
+                                     : NBPG_SENT;
             if (a.nseeds > 1)
                 for (uint32_t j = 0; j < a.nseeds; ++j)
                     sid = lane == j ? a.sp[j] : sid;

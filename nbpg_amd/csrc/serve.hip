@@ -144,9 +144,12 @@ static int launch_slot(nbpg_server *sv, Slot &s)
     HIP_TRY(hipMemcpyAsync(s.dq, s.hq, (size_t)off * sv->stride, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dL, s.hL, off * 4ull, hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.dk, s.hk, off * 4ull, hipMemcpyHostToDevice, s.stream));
+    if (sv->cfg.seed_nearest)
+        NBPG_TRY(start_select(sv->ix, s.dq, off, sv->stride, s.dids, kmax, s.stream));
     NBPG_TRY(search_fast(sv->ix, s.dq, off, sv->stride, sv->cfg.start_points[0],
                          sv->cfg.start_points, sv->cfg.sp_num, 0, 0, s.dL, s.dk, Lmax, kmax,
-                         sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr, s.counter, s.stream));
+                         sv->ix->n, &sv->cfg.fast, s.dids, s.dd, nullptr, s.counter, s.stream,
+                         2, nullptr, 0, false, sv->cfg.seed_nearest != 0));
     HIP_TRY(hipMemcpyAsync(s.hids, s.dids, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipMemcpyAsync(s.hd, s.dd, (size_t)off * kmax * 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipEventRecord(s.ev, s.stream));
@@ -254,6 +257,10 @@ int nbpg_server_create(nbpg_server **out, const nbpg_index *ix, const nbpg_serve
     }
     if (cfg->sp_num > 1 && cfg->max_L > 4 * WAVE)
         return set_error(NBPG_EUNSUP, "several start points need max_L <= 256");
+    if (cfg->seed_nearest && (cfg->max_L > 4 * WAVE || ix->n_seeds == 0 ||
+                              !start_select_supported(ix)))
+        return set_error(NBPG_EINVAL, "nearest-seed starts need max_L <= 256 and the index's "
+                                      "seed rows (u8 / i8 rows of a multiple of 16 bytes)");
     nbpg_server *sv = new nbpg_server();
     sv->ix = ix;
     sv->cfg = *cfg;

@@ -729,3 +729,91 @@ def test_concurrent_searches_on_streams(gpu, small, kind):
             assert torch.equal(a, b), ("graph", j)
         g.close()
     ix.close()
+
+
+def _nearest_seed(base, q, seeds, dtype, mips):
+    """each query's nearest seed by exact integer distance, ties to the lower
+    id (seeds ascending): the restatement the device pass must equal"""
+    s = np.sort(seeds)
+    if dtype == "i8":
+        b, qq = base[s].view(np.int8).astype(np.int64), q.view(np.int8).astype(np.int64)
+    else:
+        b, qq = base[s].astype(np.int64), q.astype(np.int64)
+    d = -(qq @ b.T) if mips else ((qq[:, None, :] - b[None, :, :]) ** 2).sum(-1)
+    return s[np.argmin(d, axis=1)].astype(np.uint32)
+
+
+@pytest.mark.parametrize("dtype,mips", [("u8", False), ("i8", False), ("i8", True), ("u8", True)])
+def test_nearest_seed_starts(gpu, small, dtype, mips):
+    """NBPG_SEARCH_SEED_NEAREST: every query starts from its nearest seed row
+    (exact distances, ties to the lower id, 300 seeds: a padded last tile),
+    and returns exactly what the same FAST search returns when started from
+    that seed alone (ids, distances, counters), in both metrics."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    rng = np.random.default_rng(41)
+    if dtype == "i8":
+        base = base[:, :64] ^ 0x80  # 64-byte signed rows around zero
+        q = q[:, :64] ^ 0x80
+        dt, R = N.I8, 32
+    else:
+        dt, R = N.U8, 32
+    ix = make_index(N, base, dt, R, rows)
+    if mips:
+        ix.set_metric(N.METRIC_MIPS)
+    seeds = rng.choice(base.shape[0], 300, replace=False).astype(np.uint32)
+    ix.set_seeds(seeds)
+    qd = dev_rows(N, q, ix.row_stride)
+    mode = N.SEARCH_FAST | N.SEARCH_SEED_NEAREST
+    for L in (16, 64):
+        ids, d, st = N.search(ix, qd, L, 10, mode=mode)
+        ids, d, st = ids.cpu().numpy(), d.cpu().numpy(), st.cpu().numpy()
+        want_start = _nearest_seed(base, q, seeds, dtype, mips)
+        for s in np.unique(want_start):
+            sel = np.nonzero(want_start == s)[0]
+            qs = qd[torch.from_numpy(sel).cuda()]
+            wi, wd, ws = N.search(ix, qs, L, 10, mode=N.SEARCH_FAST, start_points=[int(s)])
+            assert (wi.cpu().numpy() == ids[sel]).all(), (dtype, mips, L, int(s))
+            assert (wd.cpu().numpy() == d[sel]).all(), (dtype, mips, L, int(s))
+            assert (ws.cpu().numpy() == st[sel]).all(), (dtype, mips, L, int(s))
+    ix.close()
+
+
+def test_nearest_seed_starts_errors(gpu, small):
+    """refused: duplicate or out-of-range seeds, too many, searches without
+    seeds (also after new rows drop them), with a visited list, k = 0 or a
+    4-bit index"""
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qd = dev_rows(N, q, ix.row_stride)
+    mode = N.SEARCH_FAST | N.SEARCH_SEED_NEAREST
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 10, mode=mode)  # no seeds yet
+    with pytest.raises(N.NbpgError):
+        ix.set_seeds([3, 5, 3])
+    with pytest.raises(N.NbpgError):
+        ix.set_seeds([base.shape[0]])
+    with pytest.raises(N.NbpgError):
+        ix.set_seeds(np.arange(N.MAX_SEEDS + 1))
+    ix.set_seeds([7, 1, 4000])
+    ids, _, _ = N.search(ix, qd, 16, 10, mode=mode)
+    assert (ids.cpu().numpy().view(np.uint32) < base.shape[0]).all()
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 10, mode=mode, visited=True)
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 0, mode=mode)
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 10, mode=N.SEARCH_REF | N.SEARCH_SEED_NEAREST)
+    ix.set_points(base[:10])  # new rows drop the seeds
+    with pytest.raises(N.NbpgError):
+        N.search(ix, qd, 16, 10, mode=mode)
+    ix.set_seeds([])
+    ix.close()
+    rng = np.random.default_rng(3)
+    b4 = rng.integers(0, 256, (500, 32), dtype=np.uint8)
+    ix4 = make_index(N, b4, N.U4X2, 8)
+    with pytest.raises(N.NbpgError):
+        ix4.set_seeds([1, 2])
+    ix4.close()

@@ -346,7 +346,7 @@ def run_pca(args, log):
 def run_traffic(args, log):
     """HBM traffic per launch of every FAST launch shape the headline's
     autotuner can pick, for a rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE run:
-    the headline's data, device-built graph, 64 k-means start points and 12
+    the headline's data, device-built graph, start points (--start) and 12
     query batches; per shape (waves per group, expansion width, filter size,
     row split; groups = as many as the device holds) the smallest beam width
     with recall@10 >= 0.95 on batch 0, then, after every shape's beam width
@@ -373,7 +373,17 @@ def run_traffic(args, log):
                                          ix.row_stride)))
     gt_ids, gt_d = N.compgt(base_d, torch.from_numpy(q).cuda(), 100, N.U8, N.GT_MFMA)
     gt, gtd = gt_ids.cpu().numpy().view(np.uint32), gt_d.cpu().numpy()
-    seeds = N.pick_seeds(ix, 64)
+    # the headline's start points (bench.py --start): each query's nearest of
+    # 4096 sampled seed rows, or 64 k-means seeds shared by every query
+    seeds, fmode = None, N.SEARCH_FAST
+    if args.start == "nearest":
+        ix.set_seeds(np.sort(np.random.default_rng(123).choice(
+            n, min(args.seed_rows, N.MAX_SEEDS, n), replace=False)).astype(np.uint32))
+        fmode |= N.SEARCH_SEED_NEAREST
+        start = "nearest of %d seed rows" % min(args.seed_rows, N.MAX_SEEDS, n)
+    else:
+        seeds = N.pick_seeds(ix, args.seeds)
+        start = "%d k-means seeds" % len(seeds)
     sp = AT.DEFAULT_SPACE
     shapes = []
     for wpg, w, fl, rs in itertools.product(sp["waves_per_group"], sp["expand_width"],
@@ -382,7 +392,7 @@ def run_traffic(args, log):
         for L in AT.DEFAULT_L:
             if L < 10:
                 continue
-            ids = N.search(ix, q_ds[0], L, 10, mode=N.SEARCH_FAST, fast=cfg, stats=False,
+            ids = N.search(ix, q_ds[0], L, 10, mode=fmode, fast=cfg, stats=False,
                            start_points=seeds)[0]
             r = AT.recall_with_ties(ids.cpu().numpy().view(np.uint32), gt, gtd, 10)
             if r >= args.target:
@@ -393,12 +403,12 @@ def run_traffic(args, log):
     launches = []
     for cfg, L in shapes:
         for b in range(B):
-            N.search(ix, q_ds[b], L, 10, mode=N.SEARCH_FAST, fast=cfg, start_points=seeds)
+            N.search(ix, q_ds[b], L, 10, mode=fmode, fast=cfg, start_points=seeds)
         torch.cuda.synchronize()
         launches.append({"fast_config": list(cfg), "L": L})
     ix.close()
     return {"workload": "traffic", "n": n, "nq": nq, "dims": int(base.shape[1]),
-            "query_batches": B, "launches": launches}
+            "query_batches": B, "start": start, "launches": launches}
 
 
 # ---------------------------------------------------------------------- refgraph
