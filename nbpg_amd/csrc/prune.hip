@@ -126,7 +126,8 @@ void prune_mfma_kernel(PruneArgs a)
     __shared__ uint64_t keys_all[PRUNE_MFMA_WAVES][PM_MAXC];
     __shared__ __align__(16) PruneMfmaLds lds_all[PRUNE_MFMA_WAVES];
     const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x / WAVE;
+    // wave-uniform to the compiler, so the scan's branches stay scalar
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     uint64_t *keys = keys_all[wv];
     const uint64_t t = (uint64_t)blockIdx.x * PRUNE_MFMA_WAVES + wv;
     if (t >= a.n_nodes)
@@ -169,7 +170,7 @@ void prune_mfma_kernel(PruneArgs a)
         return;
     }
     wave_sync();
-    const uint32_t m = wave_sort_candidates(keys, n, p, lane);
+    const uint32_t m = __builtin_amdgcn_readfirstlane(wave_sort_candidates(keys, n, p, lane));
     const double alpha = a.alphas ? (double)a.alphas[t] : a.alpha;
     uint32_t comps = 0, sel;
     if (alpha <= 1e-5 && a.cos_angle <= 1e-5) // prune.h:164-169, no distances

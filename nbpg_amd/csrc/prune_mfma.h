@@ -57,22 +57,23 @@ struct PruneMfmaLds {
     int32_t u[PM_MAXC];  // c(j) - T(j); with the angle test: d(p, j) as float bits
 };
 
-// fragment of candidate row `row` (nullptr: a zero fragment) for k-step s and
-// lane group kq, as signed bytes (see above)
+// fragment of candidate row `row` for k-step s and lane group kq < 4, as
+// signed bytes (see above); zero past the row's end (rows that fill every
+// k-step need no test)
 template <int DT, int NCH>
 __device__ __forceinline__ pm_v4i prune_frag(const uint8_t *__restrict__ row, int s, uint32_t kq)
 {
     pm_v4i v = {0, 0, 0, 0};
     if (prune_mfma_shape<DT, NCH>::nib) {
         const uint32_t off = 32 * s + 8 * kq; // 8 packed bytes = 16 nibbles
-        if (row && off < 16 * NCH) {
+        if (16 * NCH >= 32 * prune_mfma_shape<DT, NCH>::KS || off < 16 * NCH) {
             const uint2 u = *reinterpret_cast<const uint2 *>(row + off);
             v = pm_v4i{(int)(u.x & 0x0F0F0F0Fu), (int)((u.x >> 4) & 0x0F0F0F0Fu),
                        (int)(u.y & 0x0F0F0F0Fu), (int)((u.y >> 4) & 0x0F0F0F0Fu)};
         }
     } else {
         const uint32_t off = 64 * s + 16 * kq;
-        if (row && off < 16 * NCH) {
+        if (16 * NCH >= 64 * prune_mfma_shape<DT, NCH>::KS || off < 16 * NCH) {
             const uint4 u = *reinterpret_cast<const uint4 *>(row + off);
             const uint32_t xm = DT == NBPG_U8 ? 0x80808080u : 0u;
             v = pm_v4i{(int)(u.x ^ xm), (int)(u.y ^ xm), (int)(u.z ^ xm), (int)(u.w ^ xm)};
@@ -139,35 +140,44 @@ __device__ uint32_t wave_prune_mfma(const uint64_t *keys, uint32_t m, uint32_t p
     const uint32_t sh = mips ? 0u : 1u; // g = 1 << sh
     const double scale = (double)dot_traits<DT>::scale;
 
-    // candidate rows -> fragments (all loads issued before the first use)
+    // candidate rows -> fragments (all loads issued before the first use).
+    // Lanes past the last candidate load the centre's row instead: their
+    // columns are never live and their rows never selected, so the values
+    // are never read, and every lane's load is then unconditional.
     pm_v4i F[PM_TILES][KS];
 #pragma unroll
     for (int t = 0; t < PM_TILES; ++t) {
         const uint32_t c = 16 * t + col;
-        const uint8_t *row = nullptr;
-        if ((uint32_t)t < ntiles && c < m)
-            row = vecs + (uint64_t)key_id(keys[c]) * stride;
+        const uint8_t *row = vecs + (uint64_t)(c < m ? key_id(keys[c]) : p) * stride;
 #pragma unroll
         for (int s = 0; s < KS; ++s)
             F[t][s] = prune_frag<DT, NCH>(row, s, kq);
     }
-    // self terms: each candidate's four lane groups summed over lanes l ^ 16, l ^ 32
+    // self terms: each candidate's four lane groups summed over lanes l ^ 16,
+    // l ^ 32 (one copy per metric, so the products take no per-word select)
+    auto self_terms = [&](auto IP) {
+        constexpr bool ip = decltype(IP)::value;
 #pragma unroll
-    for (int t = 0; t < PM_TILES; ++t) {
-        int acc = 0;
+        for (int t = 0; t < PM_TILES; ++t) {
+            int acc = 0;
 #pragma unroll
-        for (int s = 0; s < KS; ++s)
+            for (int s = 0; s < KS; ++s)
 #pragma unroll
-            for (int w = 0; w < 4; ++w)
-                acc = __builtin_amdgcn_sdot4(F[t][s][w], mips ? 0x01010101 : F[t][s][w], acc, false);
-        acc += __shfl_xor(acc, 16);
-        acc += __shfl_xor(acc, 32);
-        int32_t ct = acc;
-        if (mips)
-            ct = DT == NBPG_U8 ? -(128 * acc + 8192 * (int32_t)(16 * NCH)) : 0;
-        if (kq == 0)
-            S.ct[16 * t + col] = ct;
-    }
+                for (int w = 0; w < 4; ++w)
+                    acc = __builtin_amdgcn_sdot4(F[t][s][w], ip ? 0x01010101 : F[t][s][w], acc,
+                                                 false);
+            acc += __shfl_xor(acc, 16);
+            acc += __shfl_xor(acc, 32);
+            const int32_t ct =
+                !ip ? acc : (DT == NBPG_U8 ? -(128 * acc + 8192 * (int32_t)(16 * NCH)) : 0);
+            if (kq == 0)
+                S.ct[16 * t + col] = ct;
+        }
+    };
+    if (mips)
+        self_terms(std::true_type());
+    else
+        self_terms(std::false_type());
     wave_sync();
     // per candidate: w(j) = c(j) - T(j) (or, with the angle test, d(p, j))
     for (uint32_t j = lane; j < m; j += WAVE) {
@@ -189,6 +199,11 @@ __device__ uint32_t wave_prune_mfma(const uint64_t *keys, uint32_t m, uint32_t p
             live[4 * w + f] = (uint32_t)(b >> (16 * f)) & 0xFFFFu;
     }
     wave_sync();
+    // this lane's column term of every tile, read once
+    int32_t wcol[PM_TILES];
+#pragma unroll
+    for (int tc = 0; tc < PM_TILES; ++tc)
+        wcol[tc] = S.u[16 * tc + col];
     // prune bits of row tile tr (rows 4 kq + i) against column tile tc; `ncr`
     // holds -c(r) per row (FAR rows never prune)
     auto block = [&](int tr, int tc, const pm_v4i &ncr, const pm_v4i &dpr, bool (&pr)[4],
@@ -198,7 +213,7 @@ __device__ uint32_t wave_prune_mfma(const uint64_t *keys, uint32_t m, uint32_t p
         for (int s = 0; s < KS; ++s)
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(F[tr][s], F[tc][s], acc, 0, 0, 0);
         // this lane's column: w(j) (or d(p, j) bits and c(j))
-        const int32_t wj = S.u[16 * tc + col];
+        const int32_t wj = ANGLE ? S.u[16 * tc + col] : wcol[tc];
         int32_t y[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -241,17 +256,25 @@ __device__ uint32_t wave_prune_mfma(const uint64_t *keys, uint32_t m, uint32_t p
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 b[i] = ballot(pr[i]);
+            // row r's bits over the tile in lane r (r < 16), read back by lane
+            // index in the scan below
+            uint32_t rmask;
+            {
+                const uint32_t r = lane & 15u, ri = r & 3u;
+                const uint64_t b01 = (ri & 1u) ? b[1] : b[0], b23 = (ri & 1u) ? b[3] : b[2];
+                const uint64_t bb = (ri & 2u) ? b23 : b01;
+                rmask = (uint32_t)(bb >> (16 * (r >> 2)));
+            }
             // the tile's live rows in order (prune.h:174-206)
             uint32_t todo = live[tr], ssel = 0;
-            const uint32_t sel0 = sel;
-            while (todo != 0 && sel < R) {
+            const uint32_t sel0 = sel, room = R - sel; // room > 0
+            while (todo != 0) {
                 const uint32_t rl = (uint32_t)__builtin_ctz(todo);
                 ssel |= 1u << rl;
-                ++sel;
-                const uint32_t i = rl & 3u;
-                const uint64_t bi = i == 0 ? b[0] : i == 1 ? b[1] : i == 2 ? b[2] : b[3];
-                todo &= ~(uint32_t)(bi >> (16 * (rl >> 2)));
+                todo &= ~(uint32_t)__builtin_amdgcn_readlane((int)rmask, (int)rl);
                 todo &= ~((2u << rl) - 1u); // rows up to rl are done
+                if (++sel - sel0 == room)
+                    break;
             }
             live[tr] = 0;
             // the selected rows, in order, to the output
@@ -269,6 +292,8 @@ __device__ uint32_t wave_prune_mfma(const uint64_t *keys, uint32_t m, uint32_t p
                 for (int i = 0; i < 4; ++i)
                     ncm[i] = ((ssel >> (4 * kq + i)) & 1u) ? ncr[i] : -FAR;
                 auto column = [&](int tc) {
+                    if (live[tc] == 0) // every candidate of the tile is pruned already
+                        return;
                     bool pc[4];
                     int32_t ym = 0, wj = 0;
                     block(tr, tc, ncm, dpr, pc, ym, wj);
