@@ -6,7 +6,10 @@
 //                   cap VL), existing neighbours the filter has not seen
 //                   (nsw.h:150-161), merge_union with the visited list capped
 //                   at VL (nsw.h:162-165, utilities.h:128-165),
-//                   sort_candidates + RobustPrune (nsw.h:166-170)
+//                   sort_candidates + RobustPrune (nsw.h:166-170); lists of
+//                   up to 256 rows of up to 128 bytes are handed to
+//   build_prune_kernel  RobustPrune on int8 matrix cores (same survivors),
+//                   one wavefront per inserted node
 //   forward_kernel  each node's row := its pruned list (nsw.h:416-429)
 //   edges + sorts   reverse edges (target, dist, source), ordered by target
 //                   then (dist, source) with two stable radix sorts, grouped
@@ -35,6 +38,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "prune_core.h"
+#include "prune_mfma.h"
 #include "ref_search.h"
 
 namespace nbpg {
@@ -74,6 +78,10 @@ struct InsertArgs {
     uint32_t *p_ids;
     float *p_d;
     uint32_t *p_cnt;
+    // matrix-core prune hand-off (or NULL): per insert, PM_MAXC sorted keys
+    // and their count, NBPG_SENT where the insert pruned in place
+    uint64_t *pm_keys;
+    uint32_t *pm_cnt;
 };
 
 __host__ __device__ inline uint64_t insert_wave_bytes(uint32_t L, uint32_t R, uint32_t VL)
@@ -180,14 +188,59 @@ __global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
         const uint32_t mm = wave_sort_candidates(cp, nm, index, lane);
         const float af = a.alphas ? a.alphas[index] : a.alpha_f;
         const float cf = a.coss ? a.coss[index] : a.cos_f;
-        uint32_t comps = 0;
-        const uint32_t sel =
-            wave_prune<DT, NCH>(cp, mm, index, a.R, (double)af, (double)cf, a.vecs, a.stride,
-                                a.norms, a.p_ids + j * a.R, a.p_d + j * a.R, comps, lane, a.mips);
-        if (lane == 0)
-            a.p_cnt[j] = sel;
+        if (a.pm_keys && mm <= PM_MAXC && ((double)af > 1e-5 || (double)cf > 1e-5)) {
+            // the sorted list goes to the matrix-core prune (build_prune_kernel)
+            for (uint32_t i = lane; i < mm; i += WAVE)
+                a.pm_keys[j * PM_MAXC + i] = cp[i];
+            if (lane == 0)
+                a.pm_cnt[j] = mm;
+        } else {
+            uint32_t comps = 0;
+            const uint32_t sel = wave_prune<DT, NCH>(cp, mm, index, a.R, (double)af, (double)cf,
+                                                     a.vecs, a.stride, a.norms, a.p_ids + j * a.R,
+                                                     a.p_d + j * a.R, comps, lane, a.mips);
+            if (lane == 0) {
+                a.p_cnt[j] = sel;
+                if (a.pm_cnt)
+                    a.pm_cnt[j] = NBPG_SENT;
+            }
+        }
         wave_sync();
     }
+}
+
+#define BUILD_PRUNE_WAVES 4
+
+// the inserts' lists that insert_kernel handed off (pm_cnt != NBPG_SENT):
+// limitedSimilarityPrune on matrix cores (wave_prune_mfma) with the node's
+// alpha and cos (nsw.h:170, 297-324); lists arrive sorted and deduplicated
+template <int DT, int NCH, bool ANGLE>
+__global__ __launch_bounds__(64 * BUILD_PRUNE_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
+void build_prune_kernel(InsertArgs a)
+{
+    __shared__ uint64_t keys_all[BUILD_PRUNE_WAVES][PM_MAXC];
+    __shared__ __align__(16) PruneMfmaLds lds_all[BUILD_PRUNE_WAVES];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const uint64_t j = (uint64_t)blockIdx.x * BUILD_PRUNE_WAVES + wv;
+    if (j >= a.bs)
+        return;
+    const uint32_t mm = __builtin_amdgcn_readfirstlane(a.pm_cnt[j]);
+    if (mm > PM_MAXC) // pruned in place
+        return;
+    uint64_t *keys = keys_all[wv];
+    for (uint32_t i = lane; i < mm; i += WAVE)
+        keys[i] = a.pm_keys[j * PM_MAXC + i];
+    wave_sync();
+    const uint32_t index = a.inserts[a.floor + j];
+    const double af = (double)(a.alphas ? a.alphas[index] : a.alpha_f);
+    const double cf = (double)(a.coss ? a.coss[index] : a.cos_f);
+    const uint32_t sel =
+        wave_prune_mfma<DT, NCH, ANGLE>(keys, mm, index, a.R, af, cf, a.vecs, a.stride,
+                                        lds_all[wv], a.p_ids + j * a.R, a.p_d + j * a.R, lane,
+                                        a.mips);
+    if (lane == 0)
+        a.p_cnt[j] = sel;
 }
 
 __global__ void forward_kernel(const uint32_t *__restrict__ inserts, uint64_t floor, uint64_t bs,
@@ -369,6 +422,8 @@ struct BuildCtx {
     size_t sort_bytes = 0;
     float *popular = nullptr, *alphas = nullptr, *coss = nullptr;
     uint8_t *gscratch = nullptr;
+    uint64_t *pm_keys = nullptr; // matrix-core prune hand-off (row shapes that allow it)
+    uint32_t *pm_cnt = nullptr;
     uint64_t wave_bytes = 0;
     bool use_lds = true;
     uint32_t ins_grid = 0;
@@ -395,6 +450,8 @@ static void build_free(BuildCtx &c)
     hipFree(c.alphas);
     hipFree(c.coss);
     hipFree(c.gscratch);
+    hipFree(c.pm_keys);
+    hipFree(c.pm_cnt);
 }
 
 static int download_graph(nbpg_index *ix, std::vector<uint32_t> &nbr, std::vector<uint32_t> &deg)
@@ -427,6 +484,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     HIP_TRY(hipMemcpyAsync(c.d_inserts, inserts.data(), m * 4, hipMemcpyHostToDevice, c.s));
     float alpha_f = 1.0f, cos_f = (float)bp->min_cos_angle; // nsw.h:297-324
     const float *alphas = nullptr, *coss = nullptr;
+    double max_cos = 0.0; // the largest per-node cos (PopularPrune)
     if (pop_mode == POP_FIXED_MIN) {
         alpha_f = (float)bp->max_alpha;
         cos_f = (float)bp->max_cos_angle;
@@ -439,6 +497,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
             const float cof = (c.h_popular[i] - c.maxPop) / (c.minPop - c.maxPop);
             al[i] = minA + cof * (maxA - minA);
             co[i] = minC + cof * (maxC - minC);
+            max_cos = std::max(max_cos, (double)co[i]);
         }
         HIP_TRY(hipMemcpyAsync(c.alphas, al.data(), c.N * 4, hipMemcpyHostToDevice, c.s));
         HIP_TRY(hipMemcpyAsync(c.coss, co.data(), c.N * 4, hipMemcpyHostToDevice, c.s));
@@ -479,6 +538,10 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
     a.p_ids = c.p_ids;
     a.p_d = c.p_d;
     a.p_cnt = c.p_cnt;
+    a.pm_keys = c.pm_keys;
+    a.pm_cnt = c.pm_cnt;
+    // the angle test is evaluated when some node's cos exceeds 1e-5 (prune.h:200)
+    const bool pm_angle = (coss ? max_cos : (double)cos_f) > 1e-5;
     const uint32_t nsp = sp_mode == SP_MERGE ? bp->num_split : 1;
 
     ReverseArgs ra;
@@ -530,6 +593,17 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
                 const uint32_t grid = (uint32_t)std::min<uint64_t>(bs, c.ins_grid);
                 hipLaunchKernelGGL((insert_kernel<DT.value, NCH.value, false>), dim3(grid),
                                    dim3(64), 0, c.s, a);
+            }
+            if constexpr (prune_mfma_shape<DT.value, NCH.value>::ok) {
+                if (a.pm_keys) {
+                    const dim3 pg((uint32_t)((bs + BUILD_PRUNE_WAVES - 1) / BUILD_PRUNE_WAVES));
+                    if (pm_angle)
+                        hipLaunchKernelGGL((build_prune_kernel<DT.value, NCH.value, true>), pg,
+                                           dim3(64 * BUILD_PRUNE_WAVES), 0, c.s, a);
+                    else
+                        hipLaunchKernelGGL((build_prune_kernel<DT.value, NCH.value, false>), pg,
+                                           dim3(64 * BUILD_PRUNE_WAVES), 0, c.s, a);
+                }
             }
         });
         NBPG_TRY(rc);
@@ -644,6 +718,13 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
         c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 8);
         ok = hipMalloc(&c.gscratch, (uint64_t)c.ins_grid * c.wave_bytes) == hipSuccess;
     }
+    bool pm_shape = false;
+    dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
+        pm_shape = prune_mfma_shape<DT.value, NCH.value>::ok;
+    });
+    if (pm_shape && ok)
+        ok = hipMalloc(&c.pm_keys, maxb * PM_MAXC * 8) == hipSuccess &&
+             hipMalloc(&c.pm_cnt, maxb * 4) == hipSuccess;
     if (ok) {
         size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
         rocprim::radix_sort_pairs(nullptr, b1, c.keyA, c.keyB, c.tgtA, c.tgtB, ne, 0u, 64u, c.s);

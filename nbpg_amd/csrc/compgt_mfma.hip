@@ -220,20 +220,23 @@ void mf_scan_kernel(MfArgs a)
     auto score = [&](int buf, uint64_t b0) {
         if (!active)
             return;
-#pragma unroll
-        for (int blk = 0; blk < MF_BLK / 16; ++blk) {
-            v4i b[KS];
+        v4i b[KS], init;
+        auto ldb = [&](int blk) {
 #pragma unroll
             for (int s = 0; s < KS; ++s)
                 b[s] = *reinterpret_cast<const v4i *>(
                     &rows_lds[buf][(16 * blk + col) * PITCH + 64 * s + 16 * kq]);
+            init = *reinterpret_cast<const v4i *>(&i_lds[buf][16 * blk + 4 * kq]);
+        };
+        ldb(0);
+#pragma unroll
+        for (int blk = 0; blk < MF_BLK / 16; ++blk) {
             // scan: the accumulator starts at -h (h = floor(w / 2), L2) or
             // -w (inner product), shared by every query tile, so the matrix
             // core returns c' = c - h or c - w.  e = w - 2c <= U implies
             // c' >= -floor(U / 2) (a superset that adds only e = U + 1), and
             // e = w - c <= U is c' >= -U: one compare on the largest of the
             // four values per tile
-            const v4i init = *reinterpret_cast<const v4i *>(&i_lds[buf][16 * blk + 4 * kq]);
             // chunk-major order: each accumulator's next product is issued
             // MF_QW products after its last, so none waits on its own
             // result; one wave-wide branch per sub-block
@@ -246,6 +249,9 @@ void mf_scan_kernel(MfArgs a)
 #pragma unroll
                 for (int t = 0; t < MF_QW; ++t)
                     c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(b[s], qf[t][s], c[t], 0, 0, 0);
+            // the next sub-block's fragments, in flight during this one's test
+            if (blk + 1 < MF_BLK / 16)
+                ldb(blk + 1);
             if (THRESH) {
                 const v4i w4 = *reinterpret_cast<const v4i *>(&w_lds[buf][16 * blk + 4 * kq]);
 #pragma unroll
