@@ -93,8 +93,21 @@ __host__ __device__ inline uint64_t insert_wave_bytes(uint32_t L, uint32_t R, ui
     return lo.total + 8ull * R + 8ull * c + 16;
 }
 
+// A wavefront's search state (19 KB at L 70, R 64; 27 KB at L 96) lives in
+// LDS when the whole batch fits the CUs' LDS at once (one wavefront per
+// insert), and otherwise in a global scratch slot per wavefront of a
+// persistent grid: LDS would hold 5-8 wavefronts per CU, the slots hold 32
+// (eight per SIMD, 64 VGPRs), and the search's row and adjacency gathers
+// need those wavefronts in flight more than the state needs LDS latency
+// (1M x 128, L 70: 1.65 -> 1.50 s per build; 2M x 96, L 96: 6.1 -> 3.8 s).
+// Rows of 256 bytes keep their query in 64 more registers: four wavefronts
+// per SIMD.
+__host__ __device__ constexpr uint32_t insert_waves_per_simd(int NCH) { return NCH > 8 ? 4u : 8u; }
+#define INSERT_LDS_PER_CU (160u * 1024u) // gfx950
+
 template <int DT, int NCH, bool LDS>
-__global__ __launch_bounds__(64) void insert_kernel(InsertArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(insert_waves_per_simd(NCH))))
+void insert_kernel(InsertArgs a)
 {
     extern __shared__ __align__(16) uint8_t lds_ins[];
     uint8_t *st = LDS ? lds_ins : a.gscratch + (uint64_t)blockIdx.x * a.wave_bytes;
@@ -425,8 +438,8 @@ struct BuildCtx {
     uint64_t *pm_keys = nullptr; // matrix-core prune hand-off (row shapes that allow it)
     uint32_t *pm_cnt = nullptr;
     uint64_t wave_bytes = 0;
-    bool use_lds = true;
     uint32_t ins_grid = 0;
+    uint64_t lds_batch = 0; // batches up to this size run with LDS state
     std::vector<float> h_popular;
     float minPop = 0.0f, maxPop = 1.0f;
 };
@@ -586,7 +599,7 @@ static int batch_insert(BuildCtx &c, const std::vector<uint32_t> &inserts, int p
         if (sp_mode == SP_MERGE)
             c.sp_ctr += bs * nsp;
         int rc = dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
-            if (c.use_lds) {
+            if (bs <= c.lds_batch) {
                 hipLaunchKernelGGL((insert_kernel<DT.value, NCH.value, true>), dim3((uint32_t)bs),
                                    dim3(64), (uint32_t)c.wave_bytes, c.s, a);
             } else {
@@ -701,7 +714,6 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
         maxb = 1;
     c.maxb = maxb;
     c.wave_bytes = insert_wave_bytes(c.L, c.R, c.VL);
-    c.use_lds = c.wave_bytes <= 65536;
     const uint64_t ne = maxb * c.R;
     bool ok = hipMalloc(&c.d_inserts, c.N * 4) == hipSuccess &&
               hipMalloc(&c.p_ids, ne * 4) == hipSuccess && hipMalloc(&c.p_d, ne * 4) == hipSuccess &&
@@ -714,9 +726,17 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
               hipMalloc(&c.popular, c.N * 4) == hipSuccess &&
               hipMalloc(&c.alphas, c.N * 4) == hipSuccess &&
               hipMalloc(&c.coss, c.N * 4) == hipSuccess;
-    if (!c.use_lds && ok) {
-        c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 8);
+    if (ok) {
+        uint32_t per_simd = 8;
+        dispatch_dt_nch(ix->dtype, ix->nch,
+                        [&](auto, auto NCH) { per_simd = insert_waves_per_simd(NCH.value); });
+        c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 4 * per_simd);
         ok = hipMalloc(&c.gscratch, (uint64_t)c.ins_grid * c.wave_bytes) == hipSuccess;
+        // every wavefront of the batch resident at once with its state in LDS
+        c.lds_batch = c.wave_bytes <= (uint64_t)std::min(g_dev.lds_max, 65536)
+                          ? (uint64_t)g_dev.cu * std::min<uint64_t>(INSERT_LDS_PER_CU / c.wave_bytes,
+                                                                    4ull * per_simd)
+                          : 0;
     }
     bool pm_shape = false;
     dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
