@@ -96,17 +96,12 @@ __host__ __device__ inline uint64_t insert_wave_bytes(uint32_t L, uint32_t R, ui
 // A wavefront's search state (19 KB at L 70, R 64; 27 KB at L 96) lives in
 // LDS when the whole batch fits the CUs' LDS at once (one wavefront per
 // insert), and otherwise in a global scratch slot per wavefront of a
-// persistent grid: LDS would hold 5-8 wavefronts per CU, the slots hold 32
-// (eight per SIMD, 64 VGPRs), and the search's row and adjacency gathers
-// need those wavefronts in flight more than the state needs LDS latency
-// (1M x 128, L 70: 1.65 -> 1.50 s per build; 2M x 96, L 96: 6.1 -> 3.8 s).
-// Rows of 256 bytes keep their query in 64 more registers: four wavefronts
-// per SIMD.
-__host__ __device__ constexpr uint32_t insert_waves_per_simd(int NCH) { return NCH > 8 ? 4u : 8u; }
-#define INSERT_LDS_PER_CU (160u * 1024u) // gfx950
-
+// persistent grid (ref_global_waves_per_simd): LDS would hold 5-8
+// wavefronts per CU, the slots hold 32 (1M x 128, L 70: 1.65 -> 1.50 s per
+// build; 2M x 96, L 96: 6.1 -> 3.8 s).
 template <int DT, int NCH, bool LDS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(insert_waves_per_simd(NCH))))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(LDS ? 1 : ref_global_waves_per_simd(NCH))))
 void insert_kernel(InsertArgs a)
 {
     extern __shared__ __align__(16) uint8_t lds_ins[];
@@ -729,12 +724,12 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
     if (ok) {
         uint32_t per_simd = 8;
         dispatch_dt_nch(ix->dtype, ix->nch,
-                        [&](auto, auto NCH) { per_simd = insert_waves_per_simd(NCH.value); });
+                        [&](auto, auto NCH) { per_simd = ref_global_waves_per_simd(NCH.value); });
         c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 4 * per_simd);
         ok = hipMalloc(&c.gscratch, (uint64_t)c.ins_grid * c.wave_bytes) == hipSuccess;
         // every wavefront of the batch resident at once with its state in LDS
         c.lds_batch = c.wave_bytes <= (uint64_t)std::min(g_dev.lds_max, 65536)
-                          ? (uint64_t)g_dev.cu * std::min<uint64_t>(INSERT_LDS_PER_CU / c.wave_bytes,
+                          ? (uint64_t)g_dev.cu * std::min<uint64_t>(REF_LDS_PER_CU / c.wave_bytes,
                                                                     4ull * per_simd)
                           : 0;
     }

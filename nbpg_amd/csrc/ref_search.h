@@ -72,6 +72,16 @@ __host__ __device__ inline RefLayout ref_layout(uint32_t L, uint32_t R, uint32_t
     return o;
 }
 
+// Where a wavefront's state lives.  In LDS, 160 KB per CU (gfx950) holds
+// 160 KB / state wavefronts; in a global scratch slot per wavefront, a
+// persistent grid holds 8 per SIMD (64 VGPRs; 4 for rows over 128 bytes,
+// whose query takes 64 more).  The search's steps are dependent row and
+// adjacency gathers, so wavefronts in flight count for more than LDS
+// latency once LDS holds fewer than about 10 per CU (1M x 128, 10k
+// queries, L 128 (34 KB): 12.1 -> 6.5 ms; L 64 (15 KB): equal).
+#define REF_LDS_PER_CU (160u * 1024u)
+__host__ __device__ constexpr uint32_t ref_global_waves_per_simd(int NCH) { return NCH > 8 ? 4u : 8u; }
+
 // 2^max(10, ceil(log2(L*L)) - 2) with the reference's uint32 product
 __host__ __device__ inline uint32_t ref_hash_size(uint32_t L)
 {

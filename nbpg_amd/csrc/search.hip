@@ -25,7 +25,9 @@ namespace nbpg {
 // --------------------------------------------------------------------------
 
 template <int DT, int NCH, bool LDS>
-__global__ __launch_bounds__(64) void search_ref_kernel(SearchArgs a)
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(LDS ? 1 : ref_global_waves_per_simd(NCH))))
+void search_ref_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) uint8_t lds_state[];
     uint8_t *st = LDS ? lds_state : a.gscratch + (uint64_t)blockIdx.x * a.state_bytes;
@@ -367,8 +369,15 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     const RefLayout lo = ref_layout(a.L, a.R, a.maxv, a.H);
     a.state_bytes = lo.total;
     const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
-    const bool use_lds = lo.total <= lds_budget;
-    uint64_t grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (use_lds ? 32 : 8));
+    // LDS state when LDS holds about 10 wavefronts per CU or the whole
+    // launch at once; otherwise global slots at more wavefronts per CU
+    const uint64_t lds_waves =
+        lo.total <= lds_budget ? std::min<uint64_t>(32, REF_LDS_PER_CU / lo.total) : 0;
+    const bool use_lds = lds_waves >= 10 || (lds_waves > 0 && nq <= g_dev.cu * lds_waves);
+    uint32_t per_simd = 8;
+    dispatch_dt_nch(ix->dtype, ix->nch,
+                    [&](auto, auto NCH) { per_simd = ref_global_waves_per_simd(NCH.value); });
+    uint64_t grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (use_lds ? 32 : 4 * per_simd));
     if (!use_lds && fixed) {
         grid = std::min<uint64_t>(grid, fixed_bytes / lo.total);
         if (grid == 0)
