@@ -20,7 +20,8 @@
 //      (query, base slice, lane), counted in a register: no atomics and no
 //      memory round trip in the scan.  A kept row is one 64-bit store of
 //      (partial distance as an order-preserving unsigned, row id).
-//   3. select: per query, its segments gathered and bitonic-sorted.  They
+//   3. select: per query, its segments gathered, cut to the rows up to the
+//      histogram bucket of the k-th distance, and bitonic-sorted.  They
 //      hold every row within U, so when they hold at least k rows (and no
 //      segment overflowed) the first k are the exact top-k, ties at the
 //      k-th distance included.  Otherwise the query goes on a device list ...
@@ -399,8 +400,9 @@ __global__ __launch_bounds__(256) void mf_bound_kernel(const int32_t *__restrict
 #define MF_SEL_MAX 2048
 #define MF_SEL_SEGS 2048
 
-// per query: gather its nseg segments, sort the kept rows by (partial
-// distance, id), write the first k with their distances.  A query with an
+// per query: gather its nseg segments, keep the rows that can be among the
+// first k (a histogram cut), sort those by (partial distance, id), write the
+// first k with their distances.  A query with an
 // overflowed segment, fewer than k kept rows or more than the workgroup
 // sorts goes on the redo list.  L2 keys arrive without the parity bit of
 // their partial distance (w & 1, w = |x'|^2 from nx), added back here
@@ -419,6 +421,8 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
     __shared__ uint64_t s[MF_SEL_MAX];
     __shared__ uint32_t wsum[256 / WAVE];
     __shared__ uint32_t bad;
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wlo[256 / WAVE], whi[256 / WAVE], kept_n, cut_bucket;
     const uint64_t q = blockIdx.x;
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     if (tid == 0)
@@ -486,10 +490,95 @@ __global__ __launch_bounds__(256) void mf_select_kernel(const uint64_t *__restri
             }
         start += c[j];
     }
+    // keep only the rows that can be among the first k: a 256-bucket
+    // histogram of the distance words, the first bucket at which the count
+    // reaches k, and every row up to that bucket (rows tied with the k-th
+    // share its bucket); the sort then runs on those alone
+    hist[tid] = 0u;
+    if (tid == 0)
+        kept_n = 0u;
+    __syncthreads();
+    constexpr uint32_t PER = MF_SEL_MAX / 256;
+    uint64_t v[PER];
+    uint32_t lo = ~0u, hi = 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        const uint32_t i = tid + 256 * u;
+        v[u] = i < m ? s[i] : ~0ull;
+        if (i < m) {
+            lo = min(lo, (uint32_t)(v[u] >> 32));
+            hi = max(hi, (uint32_t)(v[u] >> 32));
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    if (lane == 0) {
+        wlo[wv] = lo;
+        whi[wv] = hi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t w = 0; w < 256 / WAVE; ++w) {
+        lo = min(lo, wlo[w]);
+        hi = max(hi, whi[w]);
+    }
+    uint32_t sh = 0;
+    while (((hi - lo) >> sh) >= 256u)
+        ++sh;
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (tid + 256 * u < m)
+            atomicAdd(&hist[((uint32_t)(v[u] >> 32) - lo) >> sh], 1u);
+    __syncthreads();
+    if (wv == 0) {
+        // lane l holds buckets 4l .. 4l + 3
+        uint32_t h4[4], t = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            h4[i] = hist[4 * lane + i];
+            t += h4[i];
+        }
+        uint32_t x = t;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o)
+                x += y;
+        }
+        // x - t rows lie in earlier lanes' buckets; the lane where the count
+        // first reaches k names the cut bucket
+        uint32_t c0 = x - t, b = 256u;
+#pragma unroll
+        for (int i = 3; i >= 0; --i) {
+            // counts through bucket 4 lane + i
+            uint32_t through = c0;
+            for (int j2 = 0; j2 <= i; ++j2)
+                through += h4[j2];
+            if (through >= k)
+                b = 4 * lane + i;
+        }
+        uint32_t bm = b;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1)
+            bm = min(bm, (uint32_t)__shfl_xor((int)bm, o));
+        if (lane == 0)
+            cut_bucket = bm;
+    }
+    __syncthreads();
+    const uint32_t cb = cut_bucket;
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (tid + 256 * u < m && (((uint32_t)(v[u] >> 32) - lo) >> sh) <= cb)
+            s[atomicAdd(&kept_n, 1u)] = v[u];
+    __syncthreads();
+    const uint32_t m2 = kept_n;
     uint32_t n2 = 1;
-    while (n2 < m)
+    while (n2 < m2)
         n2 <<= 1;
-    for (uint32_t i = m + tid; i < n2; i += 256)
+    for (uint32_t i = m2 + tid; i < n2; i += 256)
         s[i] = ~0ull;
     __syncthreads();
     for (uint32_t kk = 2; kk <= n2; kk <<= 1)
