@@ -10,10 +10,11 @@
  * may run at the same time on different streams: each launch takes the
  * next of the index's 64 work counters (so at most 64 may be in flight on
  * the device at once), and a search graph owns one.  Calls that use the
- * index's workspace -- nbpg_search_rerank, NBPG_SEARCH_REF with L beyond
- * its LDS budget, prune, build, repair -- must not overlap other calls on
- * the index (issue them on one stream, or synchronise between streams); a
- * server (nbpg_server_*) keeps counters of its own.
+ * index's workspace -- nbpg_search_rerank, NBPG_SEARCH_REF when a query's
+ * state is over 16 KB (L above about 64 at R 64) and the call has more
+ * queries than LDS holds at once, prune, build, repair -- must not overlap
+ * other calls on the index (issue them on one stream, or synchronise
+ * between streams); a server (nbpg_server_*) keeps counters of its own.
  *
  * Citations are file:line in the reference tree (godpgf/NBPG), naming the
  * interface each entry point stands in for.
@@ -243,8 +244,9 @@ int nbpg_search_rerank(const nbpg_index *idx, const void *d_queries, uint64_t nq
                        uint32_t *d_out_stats, void *stream);
 /* ---- one nbpg_search captured as a HIP graph and replayed: the batched
  * driver qsearchAll (beamSearch.h:257-303) as a single graph launch per
- * batch.  create makes one eager call (argument checks, workspace growth),
- * then captures the same call; launch replays it on `stream`, reading
+ * batch.  create makes one eager call (argument checks), then captures
+ * the same call; reference-mode state beyond LDS goes to scratch the graph
+ * owns (never the index's workspace); launch replays it on `stream`, reading
  * whatever d_queries holds at that time and writing the same outputs.  The
  * index, buffers and start points are fixed at creation; the graph has its
  * own FAST work counter, which the kernel returns to zero, so the graph

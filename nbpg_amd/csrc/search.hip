@@ -319,8 +319,37 @@ struct ExactRows {
     uint32_t rerank_factor; // 0: filtering mode
 };
 
+// where a reference-mode launch of nq queries keeps its per-query state of
+// `state` bytes: LDS when LDS holds about 10 wavefronts per CU or the whole
+// launch at once, otherwise global slots at more wavefronts per CU; the
+// grid of the launch
+static void ref_plan(const nbpg_index *ix, uint64_t state, uint64_t nq, bool *use_lds,
+                     uint64_t *grid)
+{
+    const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
+    const uint64_t lds_waves =
+        state <= lds_budget ? std::min<uint64_t>(32, REF_LDS_PER_CU / state) : 0;
+    *use_lds = lds_waves >= 10 || (lds_waves > 0 && nq <= g_dev.cu * lds_waves);
+    uint32_t per_simd = 8;
+    dispatch_dt_nch(ix->dtype, ix->nch,
+                    [&](auto, auto NCH) { per_simd = ref_global_waves_per_simd(NCH.value); });
+    *grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (*use_lds ? 32 : 4 * per_simd));
+}
+
+// global scratch bytes a reference-mode search of nq queries with these
+// parameters needs (0: its state stays in LDS)
+static uint64_t ref_scratch_bytes(const nbpg_index *ix, const nbpg_query_params *qp, uint64_t nq)
+{
+    const uint32_t maxv = (uint32_t)std::min<uint64_t>(8ull * qp->L, 1u << 30);
+    const RefLayout lo = ref_layout(qp->L, ix->R, maxv, ref_hash_size(qp->L));
+    bool use_lds = true;
+    uint64_t grid = 0;
+    ref_plan(ix, lo.total, nq, &use_lds, &grid);
+    return use_lds ? 0 : grid * lo.total;
+}
+
 // fixed: caller-provided scratch (fixed_bytes) used instead of the index's
-// workspace when the per-query state does not fit in LDS; the grid shrinks
+// workspace when the per-query state does not stay in LDS; the grid shrinks
 // to what it holds
 static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, uint64_t qstride,
                       const uint32_t *sp, uint32_t sp_num, const nbpg_query_params *qp,
@@ -368,16 +397,9 @@ static int search_ref(const nbpg_index *ix, const void *d_queries, uint64_t nq, 
     }
     const RefLayout lo = ref_layout(a.L, a.R, a.maxv, a.H);
     a.state_bytes = lo.total;
-    const uint64_t lds_budget = std::min<uint64_t>((uint64_t)g_dev.lds_max, 65536);
-    // LDS state when LDS holds about 10 wavefronts per CU or the whole
-    // launch at once; otherwise global slots at more wavefronts per CU
-    const uint64_t lds_waves =
-        lo.total <= lds_budget ? std::min<uint64_t>(32, REF_LDS_PER_CU / lo.total) : 0;
-    const bool use_lds = lds_waves >= 10 || (lds_waves > 0 && nq <= g_dev.cu * lds_waves);
-    uint32_t per_simd = 8;
-    dispatch_dt_nch(ix->dtype, ix->nch,
-                    [&](auto, auto NCH) { per_simd = ref_global_waves_per_simd(NCH.value); });
-    uint64_t grid = std::min<uint64_t>(nq, (uint64_t)g_dev.cu * (use_lds ? 32 : 4 * per_simd));
+    bool use_lds = true;
+    uint64_t grid = 0;
+    ref_plan(ix, lo.total, nq, &use_lds, &grid);
     if (!use_lds && fixed) {
         grid = std::min<uint64_t>(grid, fixed_bytes / lo.total);
         if (grid == 0)
@@ -590,7 +612,8 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
                         uint64_t query_stride, const uint32_t *h_start_points, uint32_t sp_num,
                         const nbpg_query_params *qp, int mode, const nbpg_fast_config *fast,
                         uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
-                        void *stream, uint32_t *counter)
+                        void *stream, uint32_t *counter, uint8_t *ref_scratch = nullptr,
+                        uint64_t ref_scratch_bytes = 0)
 {
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
     const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
@@ -620,7 +643,8 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
     const uint32_t out_k = frontier ? qp->L : qp->k;
     if (base == NBPG_SEARCH_REF)
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, out_k,
-                          nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream));
+                          nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream),
+                          ref_scratch, ref_scratch_bytes);
     if (nearest) {
         if (out_k == 0)
             return set_error(NBPG_EINVAL, "NBPG_SEARCH_SEED_NEAREST needs k >= 1");
@@ -778,15 +802,18 @@ struct nbpg_search_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t *counter = nullptr; // the captured launch's own work counter
+    uint8_t *scratch = nullptr;  // reference mode's state beyond LDS, the graph's own
 };
 
 namespace nbpg {
 
 // captures launch(stream, counter) into a new graph with its own work
 // counter; launch is one search call that has already run eagerly
-template <typename F> static int capture_search(nbpg_search_graph **out, F &&launch)
+template <typename F>
+static int capture_search(nbpg_search_graph **out, F &&launch, uint8_t *scratch = nullptr)
 {
     nbpg_search_graph *g = new nbpg_search_graph();
+    g->scratch = scratch; // freed with the graph
     // the graph's own work counter, so replays never share one with other
     // launches on the index
     if (hipMalloc(&g->counter, 256) != hipSuccess || hipMemset(g->counter, 0, 256) != hipSuccess) {
@@ -836,15 +863,27 @@ int nbpg_search_graph_create(nbpg_search_graph **out, const nbpg_index *ix, cons
     NBPG_TRY(ensure_init());
     if (nq == 0)
         return set_error(NBPG_EINVAL, "empty batch");
-    // one eager call first: it checks the arguments and grows the index's
-    // workspace (reference mode beyond LDS), so the capture allocates nothing
+    // one eager call first: it checks the arguments, so the capture
+    // allocates nothing
     NBPG_TRY(nbpg_search(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
                          d_out_ids, d_out_dists, d_out_stats, nullptr));
     HIP_TRY(hipDeviceSynchronize());
-    return capture_search(out, [&](hipStream_t s, uint32_t *counter) {
-        return search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode, fast,
-                            d_out_ids, d_out_dists, d_out_stats, s, counter);
-    });
+    // reference mode's state beyond LDS goes to scratch the graph owns, not
+    // to the index's workspace, which a later call may reallocate
+    const int base = mode & ~(NBPG_SEARCH_FRONTIER | NBPG_SEARCH_EDGES | NBPG_SEARCH_VISITED |
+                              NBPG_SEARCH_SEED_NEAREST);
+    const uint64_t sbytes = base == NBPG_SEARCH_REF ? ref_scratch_bytes(ix, qp, nq) : 0;
+    uint8_t *scratch = nullptr;
+    if (sbytes && hipMalloc(&scratch, sbytes) != hipSuccess)
+        return set_error(NBPG_ENOMEM, "graph scratch of %llu bytes", (unsigned long long)sbytes);
+    return capture_search(
+        out,
+        [&](hipStream_t s, uint32_t *counter) {
+            return search_entry(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, mode,
+                                fast, d_out_ids, d_out_dists, d_out_stats, s, counter, scratch,
+                                sbytes);
+        },
+        scratch);
 }
 
 int nbpg_search_uneven_graph_create(nbpg_search_graph **out, const nbpg_index *ix,
@@ -889,10 +928,12 @@ int nbpg_search_graph_destroy(nbpg_search_graph *g)
         hipGraphExecDestroy(g->exec);
     if (g->graph)
         hipGraphDestroy(g->graph);
-    if (g->counter) {
-        hipDeviceSynchronize(); // a replay still running reads it
+    if (g->counter || g->scratch)
+        hipDeviceSynchronize(); // a replay still running reads them
+    if (g->counter)
         hipFree(g->counter);
-    }
+    if (g->scratch)
+        hipFree(g->scratch);
     delete g;
     return NBPG_OK;
 }

@@ -648,6 +648,32 @@ def test_search_graph_replay_equals_eager(gpu, small, mode, L):
     ix.close()
 
 
+def test_ref_search_graph_owns_its_scratch(gpu, small):
+    """A reference-mode graph whose per-query state lives outside LDS (L 256:
+    a 64 KB filter) keeps that state in scratch of its own: eager searches
+    that grow and reallocate the index's workspace between replays (more
+    queries, larger L) leave every replay equal to the eager search."""
+    import torch
+    N = gpu
+    base, q, rows = small
+    ix = make_index(N, base, N.U8, 32, rows)
+    qa = dev_rows(N, q[:100], ix.row_stride)
+    qall = dev_rows(N, q, ix.row_stride)
+    buf = qa.clone()
+    g = N.SearchGraph(ix, buf, 256, 10, mode=N.SEARCH_REF)
+    for rep, L in enumerate((300, 400)):
+        N.search(ix, qall, L, 10, mode=N.SEARCH_REF)  # grows the workspace
+        torch.cuda.synchronize()
+        got = [t.clone() for t in g.replay()]
+        torch.cuda.synchronize()
+        want = N.search(ix, qa, 256, 10, mode=N.SEARCH_REF)
+        torch.cuda.synchronize()
+        for a, b in zip(got, want):
+            assert torch.equal(a, b), (L, rep)
+    g.close()
+    ix.close()
+
+
 def test_uneven_search_graph_replay_equals_eager(gpu, small):
     """nbpg_search_uneven_graph_*: the captured uneven search replayed with new
     query rows, beam widths and k written into the captured buffers between
