@@ -94,11 +94,11 @@ __host__ __device__ inline uint64_t insert_wave_bytes(uint32_t L, uint32_t R, ui
 }
 
 // A wavefront's search state (19 KB at L 70, R 64; 27 KB at L 96) lives in
-// LDS when the whole batch fits the CUs' LDS at once (one wavefront per
-// insert), and otherwise in a global scratch slot per wavefront of a
-// persistent grid (ref_global_waves_per_simd): LDS would hold 5-8
-// wavefronts per CU, the slots hold 32 (1M x 128, L 70: 1.65 -> 1.50 s per
-// build; 2M x 96, L 96: 6.1 -> 3.8 s).
+// LDS when LDS holds ten or more wavefronts per CU or the whole batch at once
+// (one wavefront per insert), and otherwise in a global scratch slot per
+// wavefront of a persistent grid (ref_global_waves_per_simd): LDS would hold
+// 5-8 wavefronts per CU, the slots hold 32 (1M x 128, L 70: 1.65 -> 1.50 s
+// per build; 2M x 96, L 96: 6.1 -> 3.8 s).
 template <int DT, int NCH, bool LDS>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(LDS ? 1 : ref_global_waves_per_simd(NCH))))
@@ -727,11 +727,14 @@ extern "C" int nbpg_build(nbpg_index *ix, const nbpg_build_params *bp, uint64_t 
                         [&](auto, auto NCH) { per_simd = ref_global_waves_per_simd(NCH.value); });
         c.ins_grid = (uint32_t)std::min<uint64_t>(maxb, (uint64_t)g_dev.cu * 4 * per_simd);
         ok = hipMalloc(&c.gscratch, (uint64_t)c.ins_grid * c.wave_bytes) == hipSuccess;
-        // every wavefront of the batch resident at once with its state in LDS
-        c.lds_batch = c.wave_bytes <= (uint64_t)std::min(g_dev.lds_max, 65536)
-                          ? (uint64_t)g_dev.cu * std::min<uint64_t>(REF_LDS_PER_CU / c.wave_bytes,
-                                                                    4ull * per_simd)
-                          : 0;
+        // LDS state for batches that are resident at once, and for every
+        // batch when LDS holds ten or more wavefronts per CU (the rule of the
+        // reference-mode search)
+        const uint64_t lds_waves =
+            c.wave_bytes <= (uint64_t)std::min(g_dev.lds_max, 65536)
+                ? std::min<uint64_t>(REF_LDS_PER_CU / c.wave_bytes, 32)
+                : 0;
+        c.lds_batch = lds_waves >= 10 ? ~0ull : (uint64_t)g_dev.cu * lds_waves;
     }
     bool pm_shape = false;
     dispatch_dt_nch(ix->dtype, ix->nch, [&](auto DT, auto NCH) {
