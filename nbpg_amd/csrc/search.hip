@@ -607,13 +607,14 @@ static int check_search(const nbpg_index *ix, const void *d_queries, uint64_t nq
 }
 
 // nbpg_search's work; counter: the FAST launch's work counter (NULL: the
-// index's next one)
+// index's next one); ref_fixed: scratch for reference-mode state beyond LDS
+// in place of the index's workspace (NULL: the workspace)
 static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq,
                         uint64_t query_stride, const uint32_t *h_start_points, uint32_t sp_num,
                         const nbpg_query_params *qp, int mode, const nbpg_fast_config *fast,
                         uint32_t *d_out_ids, float *d_out_dists, uint32_t *d_out_stats,
-                        void *stream, uint32_t *counter, uint8_t *ref_scratch = nullptr,
-                        uint64_t ref_scratch_bytes = 0)
+                        void *stream, uint32_t *counter, uint8_t *ref_fixed = nullptr,
+                        uint64_t ref_fixed_bytes = 0)
 {
     const bool frontier = (mode & NBPG_SEARCH_FRONTIER) != 0;
     const bool edges = (mode & NBPG_SEARCH_EDGES) != 0;
@@ -644,7 +645,7 @@ static int search_entry(const nbpg_index *ix, const void *d_queries, uint64_t nq
     if (base == NBPG_SEARCH_REF)
         return search_ref(ix, d_queries, nq, query_stride, h_start_points, sp_num, qp, out_k,
                           nullptr, d_out_ids, d_out_dists, d_out_stats, as_stream(stream),
-                          ref_scratch, ref_scratch_bytes);
+                          ref_fixed, ref_fixed_bytes);
     if (nearest) {
         if (out_k == 0)
             return set_error(NBPG_EINVAL, "NBPG_SEARCH_SEED_NEAREST needs k >= 1");
